@@ -1,0 +1,244 @@
+/*
+ * fvm.h -- C ABI of the MI355X-native 2D compressible finite-volume time step.
+ *
+ * This is the drop-in boundary.  Every entry point is `extern "C"`, takes plain
+ * pointers and sizes and returns an fvm_status.  The C++ class `CFDSolver` in
+ * <fvm/solver.hpp> wraps these calls with the exact method names of the
+ * reference class (src/include/solver.cuh:22-80), so the ImGui front end
+ * (src/main.cpp) and KernelBenchmark (src/test/kernel_benchmark.cu) can link
+ * against this library instead of src/solver.cu.
+ *
+ * Threading: one handle per host thread; every handle owns one HIP stream.
+ * Errors: every call returns FVM_OK or a negative status; fvm_last_error()
+ * gives the message of the last failure on the calling thread.
+ */
+#ifndef FVM_FVM_H
+#define FVM_FVM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVM_ABI_VERSION 1
+
+typedef enum fvm_status {
+    FVM_OK = 0,
+    FVM_ERR_INVALID_ARGUMENT = -1,
+    FVM_ERR_HIP = -2,
+    FVM_ERR_OUT_OF_MEMORY = -3,
+    FVM_ERR_NOT_INITIALIZED = -4,
+    FVM_ERR_CAPACITY = -5,
+    FVM_ERR_NOT_FOUND = -6
+} fvm_status;
+
+/* Cell tags; values are ABI (src/include/common.h:105-111). */
+enum {
+    FVM_CELL_FLUID = 0,
+    FVM_CELL_SOLID = 1,
+    FVM_CELL_GHOST = 2,
+    FVM_CELL_INFLOW = 3,
+    FVM_CELL_OUTFLOW = 4
+};
+
+/* Obstacle shapes; values are ABI (src/include/common.h:39-47). */
+enum {
+    FVM_SHAPE_CIRCLE = 0,
+    FVM_SHAPE_STAR = 1,
+    FVM_SHAPE_DIAMOND = 2,
+    FVM_SHAPE_CAPSULE = 3,
+    FVM_SHAPE_TRIANGLE = 4,
+    FVM_SHAPE_STARSHIP = 5
+};
+
+/* Riemann flux (addition: the reference always uses HLLC, solver.cu:1571). */
+enum { FVM_FLUX_HLLC = 0, FVM_FLUX_AUSM_PLUS = 1 };
+/* MUSCL limiter (addition: the reference has minmod only, solver.cu:110). */
+enum { FVM_LIMITER_MINMOD = 0, FVM_LIMITER_VAN_LEER = 1 };
+/* Inviscid wall: reference slip/no-slip blend (solver.cu:2146-2173) or pure slip. */
+enum { FVM_WALL_REFERENCE = 0, FVM_WALL_SLIP = 1 };
+/* Arithmetic of the state and the update. */
+enum { FVM_PRECISION_FP32 = 0, FVM_PRECISION_FP32_KAHAN = 1, FVM_PRECISION_FP64 = 2 };
+/* Where dt comes from: params.dt (reference step()), or the CFL limit of the
+ * current state recomputed on the device before every step. */
+enum { FVM_DT_HOST = 0, FVM_DT_DEVICE_CFL = 1 };
+/* Seeded initial perturbation (addition; the reference starts from uniform flow). */
+enum { FVM_NOISE_NONE = 0, FVM_NOISE_DENSITY = 1, FVM_NOISE_TRANSVERSE_VELOCITY = 2 };
+
+/* Fields for fvm_field_to_device / readback (src/solver.cu:3527-3671, 3444-3459). */
+enum {
+    FVM_FIELD_TEMPERATURE = 0,
+    FVM_FIELD_PRESSURE = 1,
+    FVM_FIELD_DENSITY = 2,
+    FVM_FIELD_VELOCITY_MAG = 3,
+    FVM_FIELD_MACH = 4,
+    FVM_FIELD_CELL_TYPE = 5
+};
+
+/*
+ * Simulation parameters.  The first block mirrors `SimParams`
+ * (src/include/common.h:50-91) field for field, with bools and the shape enum
+ * widened to int32 so the layout is the same for C, C++ and ctypes.  The
+ * second block turns the reference's compile-time physical constants
+ * (src/solver.cu:10-14, 39-41, 52; src/include/common.h:7-19) into run-time
+ * values so that SI and nondimensional runs share one code path.  The third
+ * block selects the additions.  Initialise with fvm_params_default().
+ */
+typedef struct fvm_params {
+    /* ---- SimParams ---- */
+    int32_t nx, ny;
+    float dx, dy;
+    float domain_width, domain_height;
+    float mach, T_inf, p_inf, rho_inf, u_inf, v_inf, c_inf;
+    float obstacle_x, obstacle_y, obstacle_r, obstacle_rotation;
+    int32_t obstacle_shape;
+    float wing_rotation;
+    float cfl, dt, t_current;
+    int32_t step;
+    int32_t enable_viscosity;
+    float cfl_visc;
+    int32_t adiabatic_wall;
+    float T_wall;
+    int32_t paused;
+    /* ---- unit system (defaults: the reference's SI constants) ---- */
+    float gamma;                   /* 1.4      common.h:7            */
+    float r_gas;                   /* 287.05   common.h:8            */
+    float mu_ref;                  /* 1.716e-5 common.h:14           */
+    float t_ref;                   /* 273.15   common.h:15           */
+    float s_sutherland;            /* 110.4    common.h:16           */
+    float prandtl;                 /* 0.71     common.h:19           */
+    float min_density;             /* 1e-4     solver.cu:10          */
+    float min_pressure;            /* 1e-2     solver.cu:11          */
+    float max_temperature;         /* 5000     solver.cu:12          */
+    float min_temperature;         /* 50       solver.cu:13          */
+    float low_density_threshold;   /* 0.01     solver.cu:39          */
+    float max_velocity_limit;      /* 5000     solver.cu:40          */
+    float max_ghost_velocity;      /* 2500     solver.cu:41          */
+    float slip_reference_velocity; /* 100      solver.cu:52          */
+    /* ---- additions ---- */
+    int32_t flux_scheme;  /* FVM_FLUX_*      */
+    int32_t limiter;      /* FVM_LIMITER_*   */
+    int32_t wall_mode;    /* FVM_WALL_*      */
+    int32_t precision;    /* FVM_PRECISION_* */
+    int32_t dt_mode;      /* FVM_DT_*        */
+    int32_t noise_field;  /* FVM_NOISE_*     */
+    float noise_amplitude;
+    uint32_t noise_seed;
+    int32_t reserved[8];
+} fvm_params;
+
+/* Reference defaults (SimParams initialisers, common.h:53-91) plus SI units. */
+void fvm_params_default(fvm_params* p);
+/* SimParams::computeDerived (common.h:93-101) with the run-time gas constant. */
+void fvm_params_compute_derived(fvm_params* p);
+
+/* ------------------------------------------------------------------------ */
+/* Plain single-case solver: the CFDSolver counterpart.                      */
+/* ------------------------------------------------------------------------ */
+typedef struct fvm_solver fvm_solver;
+
+/* Handle on HIP device `device`.  Replaces the constructor (solver.cu:2874). */
+int fvm_create(int device, fvm_solver** out);
+/* Replaces the destructor / freeMemory (solver.cu:2877-2880, 2984-3095). */
+int fvm_destroy(fvm_solver* s);
+/* initialize(params): allocate for nx*ny and reset (solver.cu:3102-3111). */
+int fvm_initialize(fvm_solver* s, const fvm_params* p);
+/* resize(nx, ny): reallocate if the size changed (solver.cu:3119-3129). */
+int fvm_resize(fvm_solver* s, int nx, int ny);
+/* reset(params): init fields, mark obstacle, BC, CFL maxima (solver.cu:3136-3162). */
+int fvm_reset(fvm_solver* s, const fvm_params* p);
+/* updateObstacleGeometry(params): re-mark, zero newly exposed fluid, BC
+ * (solver.cu:3171-3196). */
+int fvm_update_obstacle(fvm_solver* s, const fvm_params* p);
+/* step(params): one SSP-RK2 step with params->dt (FVM_DT_HOST) or the device
+ * CFL dt (FVM_DT_DEVICE_CFL); advances params->t_current and params->step
+ * (solver.cu:3289-3412). */
+int fvm_step(fvm_solver* s, fvm_params* p);
+/* nsteps steps in one call; with FVM_DT_DEVICE_CFL the step sequence is
+ * replayed from a captured HIP graph and dt never leaves the device.
+ * params->t_current / step are updated at the end. */
+int fvm_advance(fvm_solver* s, fvm_params* p, int nsteps);
+/* computeStableTimeStep (solver.cu:3273-3278). */
+int fvm_compute_stable_dt(fvm_solver* s, const fvm_params* p, float* dt);
+/* queueTimeStepComputation / readTimeStepResult (solver.cu:3206-3265). */
+int fvm_queue_dt(fvm_solver* s, const fvm_params* p);
+int fvm_read_dt(fvm_solver* s, const fvm_params* p, float* dt);
+/* getCellTypes: nx*ny bytes into caller host memory (solver.cu:3432-3435). */
+int fvm_get_cell_types(fvm_solver* s, uint8_t* host);
+/* Signed distance field, nx*ny floats into caller host memory. */
+int fvm_get_sdf(fvm_solver* s, float* host);
+/* convertCellTypesToDevice / compute{Temperature,Pressure,Density,
+ * VelocityMag,Mach}ToDevice: nx*ny floats into caller device memory
+ * (solver.cu:3444-3459, 3527-3671). */
+int fvm_field_to_device(fvm_solver* s, int field, float* dev);
+/* The same fields into caller host memory. */
+int fvm_field_to_host(fvm_solver* s, int field, float* host);
+/* Conserved state (rho, rho_u, rho_v, E, rho_e), SoA, 5*nx*ny values into / from
+ * caller host memory.  The _f64 forms convert; no reference counterpart. */
+int fvm_get_state(fvm_solver* s, float* host5);
+int fvm_set_state(fvm_solver* s, const float* host5);
+int fvm_get_state_f64(fvm_solver* s, double* host5);
+/* Domain totals of (rho, rho_u, rho_v, E) summed in FP64 on the device. */
+int fvm_conservation_sums(fvm_solver* s, double out4[4]);
+/* Device-side simulated time (FP64 sum of the dt of every step since reset)
+ * and step count. */
+int fvm_get_time(fvm_solver* s, double* t, int64_t* steps);
+/* getMaxTemperature / getMaxMach (solver.cu:3417-3426). */
+int fvm_max_temperature(fvm_solver* s, float* out);
+int fvm_max_mach(fvm_solver* s, float* out);
+/* getSimulationMemoryUsage (solver.cu:3474-3514): bytes this handle holds. */
+int fvm_memory_usage(fvm_solver* s, size_t* bytes);
+/* getGPUMemoryInfo (solver.cu:3468-3471); argument order is (free, total),
+ * as in the definition. */
+int fvm_gpu_memory_info(int device, size_t* free_bytes, size_t* total_bytes);
+/* Wait for all work of this handle (replaces cudaDeviceSynchronize calls of
+ * the callers: main.cpp:365, kernel_benchmark.cu:103,113,132). */
+int fvm_synchronize(fvm_solver* s);
+/* Grid size (getNx / getNy, solver.cuh:62-63). */
+int fvm_get_size(fvm_solver* s, int* nx, int* ny);
+/* Tile shape of the stage kernel: (64,8) (64,16) (32,8) (128,4) ...; 0,0 = auto. */
+int fvm_set_tile(fvm_solver* s, int tile_x, int tile_y);
+/* Name and average duration of each kernel launched by the last fvm_advance
+ * call when timing is enabled (HIP events on the handle's stream). */
+int fvm_enable_timing(fvm_solver* s, int enable);
+int fvm_kernel_timing(fvm_solver* s, int index, char* name, int name_len, double* avg_ms, int64_t* launches);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* fvm_last_error(void);
+int fvm_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Batched engine: hundreds of independent cases advanced together.         */
+/* No reference counterpart (the reference runs one case per process).      */
+/* ------------------------------------------------------------------------ */
+typedef struct fvm_batch fvm_batch;
+
+/* mem_budget_bytes: device bytes the engine may hold for case state. */
+int fvm_batch_create(int device, size_t mem_budget_bytes, int max_cases, fvm_batch** out);
+int fvm_batch_destroy(fvm_batch* b);
+/* Admit a case (allocates from the budget, marks, initialises, applies BC).
+ * steps_total: number of steps the case runs before it is complete. */
+int fvm_batch_add_case(fvm_batch* b, const fvm_params* p, int32_t steps_total, int32_t* case_id);
+/* Bytes a case of this size would take (for admission decisions). */
+int fvm_batch_case_bytes(const fvm_params* p, size_t* bytes);
+int fvm_batch_remove_case(fvm_batch* b, int32_t case_id);
+/* Advance every active, unfinished case by up to nsteps steps; each case uses
+ * its own CFL dt (FVM_DT_DEVICE_CFL) or its params.dt (FVM_DT_HOST). */
+int fvm_batch_advance(fvm_batch* b, int32_t nsteps);
+/* Case progress: steps done, simulated time, finished flag. */
+int fvm_batch_case_status(fvm_batch* b, int32_t case_id, int32_t* steps_done, double* t, int32_t* finished);
+int fvm_batch_get_state(fvm_batch* b, int32_t case_id, float* host5);
+int fvm_batch_get_cell_types(fvm_batch* b, int32_t case_id, uint8_t* host);
+int fvm_batch_active_cases(fvm_batch* b, int32_t* n);
+int fvm_batch_bytes_in_use(fvm_batch* b, size_t* bytes);
+int fvm_batch_synchronize(fvm_batch* b);
+/* Use a captured HIP graph for fvm_batch_advance (default on). */
+int fvm_batch_use_graph(fvm_batch* b, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVM_FVM_H */

@@ -1,0 +1,186 @@
+"""ctypes binding of the CPU oracle (liboracle.so built from fvm_oracle.cpp).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product library never
+loads it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_REPO = os.path.dirname(_HERE)
+_PKG = os.path.join(_REPO, "fvm-airidynamic-simulator_amd")
+if _PKG not in sys.path:
+    sys.path.insert(0, _PKG)
+
+from fvm_amd.params import FvmParams  # noqa: E402  (the shared ABI struct)
+
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(FvmParams)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.oracle_create.argtypes = [P, ctypes.c_int]
+        L.oracle_create.restype = ctypes.c_void_p
+        L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_reset.argtypes = [ctypes.c_void_p, P]
+        L.oracle_step.argtypes = [ctypes.c_void_p, P, ctypes.c_int]
+        L.oracle_get_state.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_set_state.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_get_cell_types.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8)]
+        L.oracle_get_sdf.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.oracle_get_prims.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_maxima.argtypes = [ctypes.c_void_p, dp, dp]
+        L.oracle_stable_dt.argtypes = [ctypes.c_void_p, P]
+        L.oracle_stable_dt.restype = ctypes.c_double
+        L.oracle_conservation_sums.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = ctypes.c_int
+        L.oracle_sdf_point.argtypes = [P, ctypes.c_float, ctypes.c_float]
+        L.oracle_sdf_point.restype = ctypes.c_float
+        L.oracle_riemann_flux.argtypes = [P, dp, dp, ctypes.c_int]
+        L.oracle_limited_slope.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.oracle_limited_slope.restype = ctypes.c_double
+        L.oracle_primitives.argtypes = [P, dp, dp]
+        L.oracle_riemann_roundtrip.argtypes = [P, dp, dp]
+        L.oracle_noise_unit.argtypes = [ctypes.c_uint32, ctypes.c_int64]
+        L.oracle_noise_unit.restype = ctypes.c_double
+        L.oracle_params_default.argtypes = [P]
+        L.oracle_params_compute_derived.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def set_threads(n: int) -> int:
+    return lib().oracle_set_threads(n)
+
+
+class Oracle:
+    """One case on the CPU, FP32 (fp64=False) or FP64 (fp64=True)."""
+
+    def __init__(self, params: FvmParams, fp64: bool = True):
+        self.params = params.copy()
+        self.nx, self.ny = params.nx, params.ny
+        self.fp64 = fp64
+        self._h = lib().oracle_create(ctypes.byref(self.params), 1 if fp64 else 0)
+        if not self._h:
+            raise ValueError("oracle_create rejected the parameters")
+
+    def close(self):
+        if self._h:
+            lib().oracle_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, params: FvmParams | None = None):
+        if params is not None:
+            self.params = params.copy()
+        lib().oracle_reset(self._h, ctypes.byref(self.params))
+
+    def step(self, n: int = 1, params: FvmParams | None = None):
+        if params is not None:
+            self.params = params.copy()
+        lib().oracle_step(self._h, ctypes.byref(self.params), int(n))
+
+    def state(self) -> np.ndarray:
+        out = np.empty((5, self.ny, self.nx), dtype=np.float64)
+        lib().oracle_get_state(self._h, _dptr(out))
+        return out
+
+    def set_state(self, s: np.ndarray):
+        s = np.ascontiguousarray(s, dtype=np.float64).reshape(5, self.ny, self.nx)
+        lib().oracle_set_state(self._h, _dptr(s))
+
+    def cell_types(self) -> np.ndarray:
+        out = np.empty((self.ny, self.nx), dtype=np.uint8)
+        lib().oracle_get_cell_types(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return out
+
+    def sdf(self) -> np.ndarray:
+        out = np.empty((self.ny, self.nx), dtype=np.float32)
+        lib().oracle_get_sdf(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return out
+
+    def prims(self) -> np.ndarray:
+        out = np.empty((4, self.ny, self.nx), dtype=np.float64)
+        lib().oracle_get_prims(self._h, _dptr(out))
+        return out
+
+    def maxima(self):
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        lib().oracle_maxima(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def stable_dt(self, params: FvmParams | None = None) -> float:
+        p = params if params is not None else self.params
+        return lib().oracle_stable_dt(self._h, ctypes.byref(p))
+
+    def conservation_sums(self) -> np.ndarray:
+        out = np.zeros(4, dtype=np.float64)
+        lib().oracle_conservation_sums(self._h, _dptr(out))
+        return out
+
+
+def sdf_point(params: FvmParams, x: float, y: float) -> float:
+    return lib().oracle_sdf_point(ctypes.byref(params), x, y)
+
+
+def riemann_flux(params: FvmParams, left, right, fp64: bool = True) -> np.ndarray:
+    """Face flux of primitive states left/right = (rho, u, v, p), scheme params.flux_scheme."""
+    inp = np.ascontiguousarray(list(left) + list(right), dtype=np.float64)
+    out = np.zeros(4, dtype=np.float64)
+    lib().oracle_riemann_flux(ctypes.byref(params), _dptr(inp), _dptr(out), 1 if fp64 else 0)
+    return out
+
+
+def limited_slope(limiter: int, qm1: float, q0: float, qp1: float) -> float:
+    return lib().oracle_limited_slope(limiter, qm1, q0, qp1)
+
+
+def primitives(params: FvmParams, cons) -> np.ndarray:
+    inp = np.ascontiguousarray(cons, dtype=np.float64)
+    out = np.zeros(4, dtype=np.float64)
+    lib().oracle_primitives(ctypes.byref(params), _dptr(inp), _dptr(out))
+    return out
+
+
+def riemann_roundtrip(params: FvmParams, prim) -> np.ndarray:
+    inp = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.zeros(4, dtype=np.float64)
+    lib().oracle_riemann_roundtrip(ctypes.byref(params), _dptr(inp), _dptr(out))
+    return out
+
+
+def noise_unit(seed: int, idx: int) -> float:
+    return lib().oracle_noise_unit(seed, idx)
