@@ -1,0 +1,21 @@
+#!/bin/bash
+# Builds libfvm_amd.so (gfx950) in-tree.  Usage: build.sh [outdir]
+set -e
+HERE="$(cd "$(dirname "$0")" && pwd)"
+OUT="${1:-$HERE/../lib}"
+mkdir -p "$OUT" "$HERE/../build"
+FLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Wall -Xclang -target-feature -Xclang -packed-fp32-ops"
+pids=()
+for f in fvm_kernels fvm_engine fvm_api; do
+  hipcc $FLAGS -c "$HERE/$f.hip" -o "$HERE/../build/$f.o" 2> "$HERE/../build/$f.log" &
+  pids+=($!)
+done
+rc=0
+for i in "${!pids[@]}"; do
+  if ! wait "${pids[$i]}"; then rc=1; fi
+done
+for f in fvm_kernels fvm_engine fvm_api; do grep -v "packed-fp32-ops" "$HERE/../build/$f.log" | grep -v "^$" || true; done
+[ $rc -eq 0 ] || { echo "compile failed"; exit 1; }
+hipcc --offload-arch=gfx950 -shared -fPIC -o "$OUT/libfvm_amd.so.tmp" "$HERE/../build/fvm_kernels.o" "$HERE/../build/fvm_engine.o" "$HERE/../build/fvm_api.o"
+mv "$OUT/libfvm_amd.so.tmp" "$OUT/libfvm_amd.so"
+echo "built $OUT/libfvm_amd.so"

@@ -1,0 +1,412 @@
+// fvm_device.hpp -- device data layout and per-cell physics of the FVM step.
+//
+// Layout in HBM: every case owns SoA planes (rho, rho_u, rho_v, E, rho_e) of
+// nx*ny values, row-major idx = j*nx + i (src/solver.cu:547, 923), in up to
+// four buffer sets (A = U^n, B = U*, C = U** / blend, D = U after the first
+// viscous half step), one uint8 cell-tag plane, one FP32 signed-distance plane
+// and a compact list of ghost cells.  Per-case constants sit in a CaseDesc
+// table that kernels read with uniform (scalar) loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fvm {
+
+enum : uint8_t { CT_FLUID = 0, CT_SOLID = 1, CT_GHOST = 2, CT_INFLOW = 3, CT_OUTFLOW = 4 };
+
+// Obstacle geometry (transcendentals evaluated once on the host; FP32 always,
+// the cell mask is compared bit for bit with the oracle).
+struct Geom {
+    int shape;
+    float obs_x, obs_y, obs_r, cosR, sinR, cosW, sinW, band, dx, dy;
+    float star[20];
+};
+
+// Per-case run-time constants (host-prepared from fvm_params in the precision
+// of the state, T = float or double).
+template <class T>
+struct CaseConsts {
+    int nx, ny;
+    int visc, adiabatic, flux, limiter, wall_mode, dt_mode;
+    int noise_field;
+    uint32_t noise_seed;
+    float noise_amp;
+    float cfl, cfl_visc, min_dx, host_dt;   // dt arithmetic is FP32 as in the reference
+    T dx, dy, inv_dx, inv_dy, inv_2dx, inv_2dy;
+    T gamma, gm1, inv_gm1, R, inv_R, cp, prandtl, two_over_gm1, gm1_over_4, inv_gamma, cs_fac;
+    T min_rho, min_p, max_T, min_T, low_rho, vmax, vghost, slip_ref;
+    T mu_ref, t_ref, s_su;
+    T rho_inf, u_inf, v_inf, p_inf, e_inf, E_inf, T_wall;
+    T Rinf_x[4], Rinf_y[4];   // far-field Riemann invariants (x and y normal)
+    Geom geo;
+};
+
+struct CaseBufs {
+    void* U[4][5];        // buffer set x field; float* or double*
+    float* K[5];          // compensation carries (FP32 + Kahan variant)
+    uint8_t* ct;
+    float* sdf;
+    unsigned* slots;      // [2 slots][2]: bits of max wave speed, max mu/rho
+    double* t;            // simulated time
+    long long* steps;     // steps taken
+};
+
+template <class T>
+struct CaseDesc {
+    CaseConsts<T> c;
+    CaseBufs b;
+    int tiles_x, tiles_y;
+    int pad[2];
+};
+
+// One work item: a tile of a case (stage / diffusion kernels) or a ghost cell.
+struct WorkItem {
+    int case_id;
+    int index;
+};
+
+// Buffer roles and dt slots of one launch.
+struct Roles {
+    int in, out, blend, pre0, pre1;  // buffer-set indices (-1 = unused)
+    int slot_r, slot_w;              // dt slot read / written (0,1)
+    int zero_slot;                   // zero slot_w of every case before the writers run
+    int count_step;                  // advance the case's time and step counter
+};
+
+// ------------------------------------------------------------------ math
+template <class T> struct Num;
+
+template <> struct Num<float> {
+    // Throughput forms for the flow arithmetic: v_rcp_f32 / v_sqrt_f32 (about
+    // 1 ulp), like the reference's --use_fast_math build (CMakeLists.txt:279).
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float exp(float x) { return __expf(x); }
+    static __device__ __forceinline__ float log(float x) { return __logf(x); }
+    static __device__ __forceinline__ bool finite(float x) { return __builtin_isfinite(x); }
+};
+
+template <> struct Num<double> {
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double log(double x) { return ::log(x); }
+    static __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
+};
+
+template <class T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : (b == b ? b : a); }
+template <> __device__ __forceinline__ float tmax<float>(float a, float b) { return fmaxf(a, b); }
+template <> __device__ __forceinline__ double tmax<double>(double a, double b) { return fmax(a, b); }
+template <class T> __device__ __forceinline__ T tmin(T a, T b);
+template <> __device__ __forceinline__ float tmin<float>(float a, float b) { return fminf(a, b); }
+template <> __device__ __forceinline__ double tmin<double>(double a, double b) { return fmin(a, b); }
+
+template <class T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
+template <> __device__ __forceinline__ float tabs<float>(float a) { return fabsf(a); }
+template <> __device__ __forceinline__ double tabs<double>(double a) { return fabs(a); }
+
+// ------------------------------------------------------------------ physics
+// computePrimitivesKernelInline, src/solver.cu:862-894
+template <class T>
+__device__ __forceinline__ void primitives(const CaseConsts<T>& k, T rho, T ru, T rv, T E, T re,
+                                           T& u, T& v, T& p, T& Tt) {
+    T r = tmax(rho, (T)k.min_rho);
+    T ir = Num<T>::rcp(r);
+    u = ru * ir;
+    v = rv * ir;
+    T ke = T(0.5) * r * (u * u + v * v);
+    T eta = (E > T(0)) ? Num<T>::div(re, E) : T(1);
+    T e = (eta < T(0.001f)) ? re : E - ke;
+    p = tmax((T)k.gm1 * e, (T)k.min_p);
+    T tt = p * ir * (T)k.inv_R;
+    tt = tmin(tt, (T)k.max_T);
+    Tt = tmax(tt, (T)k.min_T);
+}
+
+// Pressure only (fluxes need rho, u, v, p; T is not used there).
+template <class T>
+__device__ __forceinline__ void primitives_uvp(const CaseConsts<T>& k, T rho, T ru, T rv, T E, T re,
+                                               T& u, T& v, T& p) {
+    T r = tmax(rho, (T)k.min_rho);
+    T ir = Num<T>::rcp(r);
+    u = ru * ir;
+    v = rv * ir;
+    T ke = T(0.5) * r * (u * u + v * v);
+    T eta = (E > T(0)) ? Num<T>::div(re, E) : T(1);
+    T e = (eta < T(0.001f)) ? re : E - ke;
+    p = tmax((T)k.gm1 * e, (T)k.min_p);
+}
+
+// minmod (solver.cu:110-115) and van Leer limited slopes
+template <class T>
+__device__ __forceinline__ T limited_slope(int limiter, T dL, T dR) {
+    if (limiter == 1) {
+        T prod = dL * dR;
+        return prod > T(0) ? Num<T>::div(T(2) * prod, dL + dR) : T(0);
+    }
+    T m = tmin(tabs(dL), tabs(dR));
+    return (dL * dR > T(0)) ? copysign(m, dL) : T(0);
+}
+
+// Physical flux in face-normal form (computeFluxX, solver.cu:1005-1014).
+template <class T>
+__device__ __forceinline__ void phys_flux(T rho, T un, T ut, T p, T E, T f[4]) {
+    T m = rho * un;
+    f[0] = m;
+    f[1] = m * un + p;
+    f[2] = m * ut;
+    f[3] = (E + p) * un;
+}
+
+// HLLC with Roe-Davis wave speeds and the Harten-type push-out
+// (hllcFluxX solver.cu:1112-1223; hllcFluxY 1315-1430 is the same with the
+// velocity components exchanged).  Branch-free selection of the side.
+template <class T>
+__device__ __forceinline__ void hllc(const CaseConsts<T>& k, T rL, T uL, T vL, T pL, T EL,
+                                     T rR, T uR, T vR, T pR, T ER, T f[4]) {
+    T irL = Num<T>::rcp(rL), irR = Num<T>::rcp(rR);
+    T cL = Num<T>::sqrt((T)k.gamma * pL * irL);
+    T cR = Num<T>::sqrt((T)k.gamma * pR * irR);
+    T sL = Num<T>::sqrt(rL), sR = Num<T>::sqrt(rR);
+    T id = Num<T>::rcp(sL + sR + T(1e-10f));
+    T u_roe = (sL * uL + sR * uR) * id;
+    T v_roe = (sL * vL + sR * vR) * id;
+    T H_roe = (sL * ((EL + pL) * irL) + sR * ((ER + pR) * irR)) * id;
+    T c2 = (T)k.gm1 * (H_roe - T(0.5) * (u_roe * u_roe + v_roe * v_roe));
+    T c_roe = (c2 > T(0)) ? Num<T>::sqrt(c2) : T(0.5) * (cL + cR);
+    T SL = tmin(uL - cL, u_roe - c_roe);
+    T SR = tmax(uR + cR, u_roe + c_roe);
+    T delta = T(0.1f) * tmax(cL, cR);
+    if (tabs(SL) < delta) SL = -delta;
+    if (tabs(SR) < delta) SR = delta;
+    T dsl = SL - uL, dsr = SR - uR;
+    T den = rL * dsl - rR * dsr;
+    T safe = copysign(tmax(tabs(den), T(1e-10f)), den);
+    T S_star = Num<T>::div(pR - pL + rL * uL * dsl - rR * uR * dsr, safe);
+    // side K: left if SL >= 0, right if SR <= 0, else by the sign of S*
+    bool useL = (SL >= T(0)) || (!(SR <= T(0)) && S_star >= T(0));
+    bool star = !(SL >= T(0)) && !(SR <= T(0));
+    T rK = useL ? rL : rR, uK = useL ? uL : uR, vK = useL ? vL : vR;
+    T pK = useL ? pL : pR, EK = useL ? EL : ER, SK = useL ? SL : SR;
+    phys_flux(rK, uK, vK, pK, EK, f);
+    if (star) {
+        T dsk = SK - uK;
+        T cd = SK - S_star;
+        if (tabs(cd) < T(1e-10f)) cd = (cd >= T(0)) ? T(1e-10f) : T(-1e-10f);
+        T icd = Num<T>::rcp(cd);
+        T rs = tmax(rK * dsk * icd, (T)k.min_rho);
+        T ps = tmax(pK + rK * dsk * (S_star - uK), (T)k.min_p);
+        T Es = (EK * dsk + ps * S_star - pK * uK) * icd;
+        f[0] += SK * (rs - rK);
+        f[1] += SK * (rs * S_star - rK * uK);
+        f[2] += SK * (rs * vK - rK * vK);
+        f[3] += SK * (Es - EK);
+    }
+}
+
+// AUSM+ (Liou 1996): split Mach/pressure polynomials, beta = 1/8, alpha = 3/16,
+// interface sound speed from the critical speed c*^2 = 2(gamma-1)/(gamma+1) H.
+template <class T>
+__device__ __forceinline__ void ausm_plus(const CaseConsts<T>& k, T rL, T uL, T vL, T pL, T EL,
+                                          T rR, T uR, T vR, T pR, T ER, T f[4]) {
+    const T beta = T(0.125), alpha = T(0.1875);
+    T HL = (EL + pL) * Num<T>::rcp(rL);
+    T HR = (ER + pR) * Num<T>::rcp(rR);
+    T csL = Num<T>::sqrt((T)k.cs_fac * HL);
+    T csR = Num<T>::sqrt((T)k.cs_fac * HR);
+    T chL = Num<T>::div(csL * csL, tmax(csL, uL));
+    T chR = Num<T>::div(csR * csR, tmax(csR, -uR));
+    T c12 = tmin(chL, chR);
+    T ic = Num<T>::rcp(c12);
+    T ML = uL * ic, MR = uR * ic;
+    T Mp, Pp, Mm, Pm;
+    if (tabs(ML) < T(1)) {
+        T a = ML + T(1), b = ML * ML - T(1);
+        Mp = T(0.25) * a * a + beta * b * b;
+        Pp = T(0.25) * a * a * (T(2) - ML) + alpha * ML * b * b;
+    } else {
+        Mp = T(0.5) * (ML + tabs(ML));
+        Pp = ML > T(0) ? T(1) : T(0);
+    }
+    if (tabs(MR) < T(1)) {
+        T a = MR - T(1), b = MR * MR - T(1);
+        Mm = -T(0.25) * a * a - beta * b * b;
+        Pm = T(0.25) * a * a * (T(2) + MR) - alpha * MR * b * b;
+    } else {
+        Mm = T(0.5) * (MR - tabs(MR));
+        Pm = MR < T(0) ? T(1) : T(0);
+    }
+    T m12 = Mp + Mm;
+    T p12 = Pp * pL + Pm * pR;
+    T mp = T(0.5) * (m12 + tabs(m12)) * c12;
+    T mm = T(0.5) * (m12 - tabs(m12)) * c12;
+    T aL = mp * rL, aR = mm * rR;
+    f[0] = aL + aR;
+    f[1] = aL * uL + aR * uR + p12;
+    f[2] = aL * vL + aR * vR;
+    f[3] = aL * HL + aR * HR;
+}
+
+// The face part of computeFluxesKernel (solver.cu:1503-1598 / 1603-1683) in
+// face-normal form.  Cells m = 0..3 are i-1, i, i+1, i+2 along the normal.
+// out = (mass, normal momentum, tangential momentum, energy, internal energy).
+template <class T>
+__device__ __forceinline__ void face_flux(const CaseConsts<T>& k, const T rho[4], const T un[4], const T ut[4],
+                                          const T p[4], const uint8_t ct[4], T out[5]) {
+    auto sg = [](uint8_t c) { return c == CT_SOLID || c == CT_GHOST; };
+    if (sg(ct[1]) || sg(ct[2])) {
+        out[0] = T(0); out[1] = p[1]; out[2] = T(0); out[3] = T(0); out[4] = T(0);
+        return;
+    }
+    bool first = sg(ct[0]) || sg(ct[3]) || rho[1] < (T)k.low_rho || rho[2] < (T)k.low_rho;
+    T h = first ? T(0) : T(0.5);
+    int lim = k.limiter;
+    T rL = rho[1] + h * limited_slope(lim, rho[1] - rho[0], rho[2] - rho[1]);
+    T uL = un[1] + h * limited_slope(lim, un[1] - un[0], un[2] - un[1]);
+    T vL = ut[1] + h * limited_slope(lim, ut[1] - ut[0], ut[2] - ut[1]);
+    T pL = p[1] + h * limited_slope(lim, p[1] - p[0], p[2] - p[1]);
+    T rR = rho[2] - h * limited_slope(lim, rho[2] - rho[1], rho[3] - rho[2]);
+    T uR = un[2] - h * limited_slope(lim, un[2] - un[1], un[3] - un[2]);
+    T vR = ut[2] - h * limited_slope(lim, ut[2] - ut[1], ut[3] - ut[2]);
+    T pR = p[2] - h * limited_slope(lim, p[2] - p[1], p[3] - p[2]);
+    rL = tmax(rL, (T)k.min_rho);
+    pL = tmax(pL, (T)k.min_p);
+    rR = tmax(rR, (T)k.min_rho);
+    pR = tmax(pR, (T)k.min_p);
+    T reL = pL * (T)k.inv_gm1, reR = pR * (T)k.inv_gm1;
+    T EL = reL + T(0.5) * rL * (uL * uL + vL * vL);
+    T ER = reR + T(0.5) * rR * (uR * uR + vR * vR);
+    T f[4];
+    if (k.flux == 1) ausm_plus(k, rL, uL, vL, pL, EL, rR, uR, vR, pR, ER, f);
+    else hllc(k, rL, uL, vL, pL, EL, rR, uR, vR, pR, ER, f);
+    out[0] = f[0]; out[1] = f[1]; out[2] = f[2]; out[3] = f[3];
+    out[4] = (uL + uR >= T(0)) ? reL * uL : reR * uR;
+}
+
+// Riemann invariants (solver.cu:680-841) in boundary-normal form.
+template <class T>
+__device__ __forceinline__ void riemann_from_prim(const CaseConsts<T>& k, T rho, T un, T ut, T p, T R[4]) {
+    T c = Num<T>::sqrt((T)k.gamma * p * Num<T>::rcp(rho));
+    R[0] = Num<T>::exp(-(T)k.gamma * Num<T>::log(rho)) * p;
+    R[1] = ut;
+    R[2] = un + c * (T)k.two_over_gm1;
+    R[3] = un - c * (T)k.two_over_gm1;
+}
+template <class T>
+__device__ __forceinline__ void prim_from_riemann(const CaseConsts<T>& k, const T R[4], T& rho, T& un, T& ut, T& p) {
+    ut = R[1];
+    un = (R[2] + R[3]) * T(0.5);
+    T c = (R[2] - R[3]) * (T)k.gm1_over_4;
+    T c2 = c * c;
+    rho = Num<T>::exp((T)k.inv_gm1 * Num<T>::log(c2 * (T)k.inv_gamma * Num<T>::rcp(R[0])));
+    p = rho * c2 * (T)k.inv_gamma;
+}
+
+// Point-wise boundary conditions of applyBoundaryConditionsKernel
+// (solver.cu:1913-2042) for a non-ghost cell; v = (rho, rho_u, rho_v, E, rho_e).
+// Returns false for interior fluid cells (nothing to do).
+template <class T>
+__device__ __forceinline__ bool bc_pointwise(const CaseConsts<T>& k, int i, int j, uint8_t c, T v[5]) {
+    if (c == CT_INFLOW || i == 0) {
+        v[0] = (T)k.rho_inf; v[1] = (T)k.rho_inf * (T)k.u_inf; v[2] = (T)k.rho_inf * (T)k.v_inf;
+        v[3] = (T)k.E_inf; v[4] = (T)k.e_inf;
+        return true;
+    }
+    if (c == CT_OUTFLOW || i == k.nx - 1) {
+        T rloc = tmax(v[0], (T)k.min_rho);
+        T u, vv, p;
+        primitives_uvp(k, v[0], v[1], v[2], v[3], v[4], u, vv, p);
+        T cl = Num<T>::sqrt((T)k.gamma * p * Num<T>::rcp(rloc));
+        T R[4];
+        riemann_from_prim(k, v[0], u, vv, p, R);
+        bool o1 = u > T(0), o3 = u + cl > T(0), o4 = u - cl > T(0);
+        R[0] = o1 ? R[0] : (T)k.Rinf_x[0];
+        R[1] = o1 ? R[1] : (T)k.Rinf_x[1];
+        R[2] = o3 ? R[2] : (T)k.Rinf_x[2];
+        R[3] = o4 ? R[3] : (T)k.Rinf_x[3];
+        T rho;
+        prim_from_riemann(k, R, rho, u, vv, p);
+        v[0] = rho; v[1] = rho * u; v[2] = rho * vv;
+        v[4] = p * (T)k.inv_gm1;
+        v[3] = v[4] + T(0.5) * rho * (u * u + vv * vv);
+        return true;
+    }
+    bool touched = false;
+    for (int side = 0; side < 2; side++) {
+        if ((side == 0 && j != 0) || (side == 1 && j != k.ny - 1)) continue;
+        touched = true;
+        T rloc = tmax(v[0], (T)k.min_rho);
+        T u, vv, p;
+        primitives_uvp(k, rloc, v[1], v[2], v[3], v[4], u, vv, p);
+        T cl = Num<T>::sqrt((T)k.gamma * p * Num<T>::rcp(rloc));
+        bool o1, o3, o4;
+        if (side == 0) { o1 = vv < T(0); o3 = vv + cl < T(0); o4 = vv - cl < T(0); }
+        else { o1 = vv > T(0); o3 = vv + cl > T(0); o4 = vv - cl > T(0); }
+        T R[4];
+        riemann_from_prim(k, rloc, vv, u, p, R);
+        R[0] = o1 ? R[0] : (T)k.Rinf_y[0];
+        R[1] = o1 ? R[1] : (T)k.Rinf_y[1];
+        R[2] = o3 ? R[2] : (T)k.Rinf_y[2];
+        R[3] = o4 ? R[3] : (T)k.Rinf_y[3];
+        prim_from_riemann(k, R, rloc, vv, u, p);
+        v[0] = rloc; v[1] = rloc * u; v[2] = rloc * vv;
+        v[4] = p * (T)k.inv_gm1;
+        v[3] = v[4] + T(0.5) * rloc * (u * u + vv * vv);
+    }
+    if (c == CT_SOLID) {
+        v[0] = (T)k.rho_inf; v[1] = T(0); v[2] = T(0); v[3] = (T)k.e_inf; v[4] = (T)k.e_inf;
+        return true;
+    }
+    return touched;
+}
+
+// Sutherland viscosity (solver.cu:2317-2319)
+template <class T>
+__device__ __forceinline__ T sutherland(const CaseConsts<T>& k, T Tv) {
+    T tr = Tv * Num<T>::rcp((T)k.t_ref);
+    return (T)k.mu_ref * (tr * Num<T>::sqrt(tr)) * ((T)k.t_ref + (T)k.s_su) * Num<T>::rcp(Tv + (T)k.s_su);
+}
+
+// CFL wave speed |v| + c of a state (computeWaveSpeedKernel, solver.cu:2729-2739)
+template <class T>
+__device__ __forceinline__ T wave_speed(const CaseConsts<T>& k, const T v[5], T* Tout = nullptr) {
+    T u, vv, p, tt;
+    primitives(k, v[0], v[1], v[2], v[3], v[4], u, vv, p, tt);
+    if (Tout) *Tout = tt;
+    T c = Num<T>::sqrt((T)k.gamma * p * Num<T>::rcp(v[0] + T(1e-10f)));
+    return Num<T>::sqrt(u * u + vv * vv) + c;
+}
+
+// dt from the maxima of the previous step (readTimeStepResult, solver.cu:3250-3265)
+template <class T>
+__device__ __forceinline__ float dt_from_slot(const CaseConsts<T>& k, const unsigned* slots, int slot) {
+    if (k.dt_mode == 0 || slots == nullptr) return k.host_dt;
+    float ms = __uint_as_float(slots[slot * 2 + 0]);
+    float mn = __uint_as_float(slots[slot * 2 + 1]);
+    float dt = k.cfl * k.min_dx / (ms + 1e-10f);
+    if (k.visc) {
+        float dtv = k.cfl_visc * k.min_dx * k.min_dx / (mn + 1e-10f);
+        dt = fminf(dt, dtv);
+    }
+    return dt;
+}
+
+// Wave-wide max (64 lanes), then across the workgroup through LDS, then one
+// atomic max on the bit pattern (non-negative floats order like their bits).
+template <int NT>
+__device__ __forceinline__ void block_max_atomic(float v, float* lds_scratch, unsigned* target) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds_scratch[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = lds_scratch[0];
+        for (int w = 1; w < NT / 64; w++) m = fmaxf(m, lds_scratch[w]);
+        if (m > 0.0f) atomicMax(target, __float_as_uint(m));
+    }
+}
+
+}  // namespace fvm

@@ -1,0 +1,993 @@
+// fvm_engine.hip -- host runtime of the FVM step (see fvm_engine.hpp).
+#include "fvm_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+#pragma clang fp contract(off)
+
+namespace fvm {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? FVM_ERR_OUT_OF_MEMORY : FVM_ERR_HIP;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------ arena
+static size_t round256(size_t x) { return (x + 255) & ~size_t(255); }
+
+Arena::~Arena() {
+    if (base_) (void)hipFree(base_);
+}
+
+int Arena::init(size_t bytes) {
+    bytes = round256(bytes);
+    FVM_HIP(hipMalloc((void**)&base_, bytes));
+    cap_ = bytes;
+    used_ = 0;
+    free_.clear();
+    live_.clear();
+    free_[0] = bytes;
+    return FVM_OK;
+}
+
+void* Arena::alloc(size_t bytes) {
+    bytes = round256(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second >= bytes) {
+            size_t off = it->first, sz = it->second;
+            free_.erase(it);
+            if (sz > bytes) free_[off + bytes] = sz - bytes;
+            live_[off] = bytes;
+            used_ += bytes;
+            return base_ + off;
+        }
+    }
+    return nullptr;
+}
+
+void Arena::release(void* p) {
+    if (!p) return;
+    size_t off = (size_t)((char*)p - base_);
+    auto it = live_.find(off);
+    if (it == live_.end()) return;
+    size_t sz = it->second;
+    live_.erase(it);
+    used_ -= sz;
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && off + sz == nx->first) {  // merge with next
+        sz += nx->second;
+        nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {  // merge with previous
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {
+            pv->second += sz;
+            return;
+        }
+    }
+    free_[off] = sz;
+}
+
+// ------------------------------------------------------------------ engine
+Engine::Engine(int device, bool fp64, bool kahan) : device_(device), fp64_(fp64), kahan_(kahan && !fp64) {
+    desc_size_ = fp64_ ? sizeof(CaseDesc<double>) : sizeof(CaseDesc<float>);
+}
+
+Engine::~Engine() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graphs();
+    for (HostCase* c : cases_) {
+        if (!c) continue;
+        if (use_arena_) arena_.release(c->mem);
+        else if (c->mem) (void)hipFree(c->mem);
+        delete c;
+    }
+    if (d_cases_) (void)hipFree(d_cases_);
+    if (d_tiles_all_) (void)hipFree(d_tiles_all_);
+    if (d_tiles_visc_) (void)hipFree(d_tiles_visc_);
+    if (d_ghosts_) (void)hipFree(d_ghosts_);
+    if (d_counts_) (void)hipFree(d_counts_);
+    if (d_stat_) (void)hipFree(d_stat_);
+    if (d_cons_) (void)hipFree(d_cons_);
+    if (h_pinned_) (void)hipHostFree(h_pinned_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Engine::init_stream() {
+    FVM_HIP(hipSetDevice(device_));
+    if (!stream_) FVM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (!d_counts_) {
+        FVM_HIP(hipMalloc((void**)&d_counts_, 8 * sizeof(int)));
+        FVM_HIP(hipMemset(d_counts_, 0, 8 * sizeof(int)));
+    }
+    if (!d_stat_) FVM_HIP(hipMalloc((void**)&d_stat_, 16 * sizeof(unsigned)));
+    if (!d_cons_) FVM_HIP(hipMalloc((void**)&d_cons_, (1024 * 4 + 8) * sizeof(double)));
+    if (!h_pinned_) FVM_HIP(hipHostMalloc((void**)&h_pinned_, 16 * sizeof(float)));
+    if (!ev0_) FVM_HIP(hipEventCreate(&ev0_));
+    if (!ev1_) FVM_HIP(hipEventCreate(&ev1_));
+    return FVM_OK;
+}
+
+int Engine::set_arena(size_t bytes) {
+    int rc = arena_.init(bytes);
+    if (rc == FVM_OK) {
+        use_arena_ = true;
+        persistent_ = true;
+    }
+    return rc;
+}
+
+int Engine::set_tile(int tx, int ty) {
+    if (!tile_supported(tx, ty)) return set_error(FVM_ERR_INVALID_ARGUMENT, "unsupported tile shape");
+    if (tx == tx_ && ty == ty_) return FVM_OK;
+    tx_ = tx;
+    ty_ = ty;
+    for (HostCase* c : cases_) {
+        if (!c) continue;
+        c->tiles_x = (c->p.nx + tx_ - 1) / tx_;
+        c->tiles_y = (c->p.ny + ty_ - 1) / ty_;
+        int rc = upload_desc(*c);
+        if (rc) return rc;
+    }
+    lists_dirty_ = true;
+    drop_graphs();
+    return FVM_OK;
+}
+
+HostCase* Engine::get(int id) {
+    if (id < 0 || id >= (int)cases_.size()) return nullptr;
+    return cases_[id];
+}
+
+int Engine::current_set(const HostCase& c) const {
+    if (c.p.enable_viscosity) return 0;
+    return phase_ ? 2 : 0;
+}
+
+int Engine::active_count() const {
+    int n = 0;
+    for (const HostCase* c : cases_)
+        if (c && c->active && !c->finished) n++;
+    return n;
+}
+
+int Engine::retire_finished() {
+    int n = 0;
+    for (HostCase* c : cases_)
+        if (c && c->active && !c->finished && c->steps_total > 0 && c->steps_done >= c->steps_total) {
+            c->finished = true;
+            n++;
+        }
+    if (n) lists_dirty_ = true;
+    return n;
+}
+
+size_t Engine::bytes_in_use() const {
+    size_t b = 0;
+    for (const HostCase* c : cases_)
+        if (c) b += c->bytes;
+    return b;
+}
+
+// Plane layout of one case inside its allocation.
+struct CaseLayout {
+    size_t n, plane, sets_off, k_off, ct_off, sdf_off, slots_off, t_off, steps_off, total;
+};
+
+static CaseLayout layout_for(const fvm_params& p, bool fp64, bool kahan, int nsets) {
+    CaseLayout L;
+    L.n = (size_t)p.nx * p.ny;
+    L.plane = round256(L.n * (fp64 ? 8 : 4));
+    size_t off = 0;
+    L.sets_off = off;
+    off += L.plane * 5 * nsets;
+    L.k_off = off;
+    if (kahan) off += round256(L.n * 4) * 5;
+    L.ct_off = off;
+    off += round256(L.n);
+    L.sdf_off = off;
+    off += round256(L.n * 4);
+    L.slots_off = off;
+    off += 256;
+    L.t_off = off;
+    off += 256;
+    L.steps_off = off;
+    off += 256;
+    L.total = off;
+    return L;
+}
+
+size_t Engine::case_bytes(const fvm_params& p, bool fp64, bool kahan, bool all_sets) {
+    int nsets = (all_sets || p.enable_viscosity) ? 4 : 3;
+    return layout_for(p, fp64, kahan, nsets).total;
+}
+
+template <class T>
+static void fill_consts(CaseConsts<T>& k, const fvm_params& p) {
+    std::memset(&k, 0, sizeof(k));
+    k.nx = p.nx;
+    k.ny = p.ny;
+    k.visc = p.enable_viscosity ? 1 : 0;
+    k.adiabatic = p.adiabatic_wall ? 1 : 0;
+    k.flux = p.flux_scheme;
+    k.limiter = p.limiter;
+    k.wall_mode = p.wall_mode;
+    k.dt_mode = p.dt_mode;
+    k.noise_field = p.noise_field;
+    k.noise_seed = p.noise_seed;
+    k.noise_amp = p.noise_amplitude;
+    k.cfl = p.cfl;
+    k.cfl_visc = p.cfl_visc;
+    k.min_dx = std::fmin(p.dx, p.dy);
+    k.host_dt = p.dt;
+    const T one = T(1);
+    k.dx = (T)p.dx;
+    k.dy = (T)p.dy;
+    k.inv_dx = one / k.dx;
+    k.inv_dy = one / k.dy;
+    k.inv_2dx = T(0.5) / k.dx;
+    k.inv_2dy = T(0.5) / k.dy;
+    k.gamma = (T)p.gamma;
+    k.gm1 = k.gamma - one;
+    k.inv_gm1 = one / k.gm1;
+    k.R = (T)p.r_gas;
+    k.inv_R = one / k.R;
+    T cv = k.R / k.gm1;
+    k.cp = k.gamma * cv;
+    k.prandtl = (T)p.prandtl;
+    k.two_over_gm1 = T(2) / k.gm1;
+    k.gm1_over_4 = k.gm1 * T(0.25);
+    k.inv_gamma = one / k.gamma;
+    k.cs_fac = T(2) * k.gm1 / (k.gamma + one);
+    k.min_rho = (T)p.min_density;
+    k.min_p = (T)p.min_pressure;
+    k.max_T = (T)p.max_temperature;
+    k.min_T = (T)p.min_temperature;
+    k.low_rho = (T)p.low_density_threshold;
+    k.vmax = (T)p.max_velocity_limit;
+    k.vghost = (T)p.max_ghost_velocity;
+    k.slip_ref = (T)p.slip_reference_velocity;
+    k.mu_ref = (T)p.mu_ref;
+    k.t_ref = (T)p.t_ref;
+    k.s_su = (T)p.s_sutherland;
+    k.rho_inf = (T)p.rho_inf;
+    k.u_inf = (T)p.u_inf;
+    k.v_inf = (T)p.v_inf;
+    k.p_inf = (T)p.p_inf;
+    k.e_inf = k.p_inf / k.gm1;
+    T ke_inf = T(0.5) * k.rho_inf * (k.u_inf * k.u_inf + k.v_inf * k.v_inf);
+    k.E_inf = k.e_inf + ke_inf;
+    k.T_wall = (T)p.T_wall;
+    // far-field Riemann invariants (computeFarfieldRiemannInvarX/Y, solver.cu:799-841)
+    T c_inf = std::sqrt(k.gamma * k.p_inf / k.rho_inf);
+    T R0 = std::exp(-k.gamma * std::log(k.rho_inf)) * k.p_inf;
+    k.Rinf_x[0] = R0;
+    k.Rinf_x[1] = k.v_inf;
+    k.Rinf_x[2] = k.u_inf + c_inf * k.two_over_gm1;
+    k.Rinf_x[3] = k.u_inf - c_inf * k.two_over_gm1;
+    k.Rinf_y[0] = R0;
+    k.Rinf_y[1] = k.u_inf;
+    k.Rinf_y[2] = k.v_inf + c_inf * k.two_over_gm1;
+    k.Rinf_y[3] = k.v_inf - c_inf * k.two_over_gm1;
+    // geometry: the per-cell cosf/sinf of the reference (solver.cu:190-210,
+    // 259-260, 330-331, 421-422, 451-452) evaluated once here
+    Geom& g = k.geo;
+    g.shape = (p.obstacle_shape >= 0 && p.obstacle_shape <= 5) ? p.obstacle_shape : 0;
+    g.obs_x = p.obstacle_x;
+    g.obs_y = p.obstacle_y;
+    g.obs_r = p.obstacle_r;
+    g.cosR = std::cos(-p.obstacle_rotation);
+    g.sinR = std::sin(-p.obstacle_rotation);
+    g.cosW = std::cos(-p.wing_rotation);
+    g.sinW = std::sin(-p.wing_rotation);
+    g.band = 1.5f * std::fmax(p.dx, p.dy);
+    g.dx = p.dx;
+    g.dy = p.dy;
+    const float kPI = 3.14159265358979323846f;
+    const int N = 5;
+    const float outerR = p.obstacle_r, innerR = p.obstacle_r * 0.38f;
+    for (int i = 0; i < N; i++) {
+        float outerAngle = 2.0f * kPI * (float)i / (float)N - kPI / 2.0f;
+        float innerAngle = outerAngle + kPI / (float)N;
+        g.star[i * 4 + 0] = outerR * std::cos(outerAngle);
+        g.star[i * 4 + 1] = outerR * std::sin(outerAngle);
+        g.star[i * 4 + 2] = innerR * std::cos(innerAngle);
+        g.star[i * 4 + 3] = innerR * std::sin(innerAngle);
+    }
+}
+
+int Engine::build_consts(HostCase& hc, const fvm_params& p, void* desc_out) {
+    if (fp64_) fill_consts(((CaseDesc<double>*)desc_out)->c, p);
+    else fill_consts(((CaseDesc<float>*)desc_out)->c, p);
+    (void)hc;
+    return FVM_OK;
+}
+
+int Engine::upload_desc(HostCase& hc) {
+    char* hd = h_descs_.data() + (size_t)hc.id * desc_size_;
+    char* base = (char*)hc.mem;
+    CaseLayout L = layout_for(hc.p, fp64_, kahan_, hc.nsets);
+    CaseBufs b;
+    std::memset(&b, 0, sizeof(b));
+    for (int s = 0; s < 4; s++)
+        for (int f = 0; f < 5; f++)
+            b.U[s][f] = (s < hc.nsets) ? (void*)(base + L.sets_off + L.plane * (s * 5 + f)) : nullptr;
+    if (kahan_)
+        for (int f = 0; f < 5; f++) b.K[f] = (float*)(base + L.k_off + round256(L.n * 4) * f);
+    b.ct = (uint8_t*)(base + L.ct_off);
+    b.sdf = (float*)(base + L.sdf_off);
+    b.slots = (unsigned*)(base + L.slots_off);
+    b.t = (double*)(base + L.t_off);
+    b.steps = (long long*)(base + L.steps_off);
+    if (fp64_) {
+        CaseDesc<double>* d = (CaseDesc<double>*)hd;
+        fill_consts(d->c, hc.p);
+        d->b = b;
+        d->tiles_x = hc.tiles_x;
+        d->tiles_y = hc.tiles_y;
+    } else {
+        CaseDesc<float>* d = (CaseDesc<float>*)hd;
+        fill_consts(d->c, hc.p);
+        d->b = b;
+        d->tiles_x = hc.tiles_x;
+        d->tiles_y = hc.tiles_y;
+    }
+    FVM_HIP(hipMemcpyAsync((char*)d_cases_ + (size_t)hc.id * desc_size_, hd, desc_size_, hipMemcpyHostToDevice, stream_));
+    FVM_HIP(hipStreamSynchronize(stream_));
+    return FVM_OK;
+}
+
+static bool params_valid(const fvm_params& p, std::string& why) {
+    if (p.nx < 3 || p.ny < 3) { why = "nx and ny must be >= 3"; return false; }
+    if ((long long)p.nx * p.ny > (1ll << 31) - 1) { why = "grid too large (nx*ny must fit in int32)"; return false; }
+    if (!(p.dx > 0) || !(p.dy > 0)) { why = "dx and dy must be positive"; return false; }
+    if (!(p.gamma > 1.0f)) { why = "gamma must exceed 1"; return false; }
+    if (!(p.rho_inf > 0) || !(p.p_inf > 0)) { why = "rho_inf and p_inf must be positive (call fvm_params_compute_derived)"; return false; }
+    if (p.flux_scheme < 0 || p.flux_scheme > 1) { why = "unknown flux scheme"; return false; }
+    if (p.limiter < 0 || p.limiter > 1) { why = "unknown limiter"; return false; }
+    if (p.dt_mode < 0 || p.dt_mode > 1) { why = "unknown dt mode"; return false; }
+    if (p.precision < 0 || p.precision > 2) { why = "unknown precision"; return false; }
+    if (p.precision == FVM_PRECISION_FP32_KAHAN && p.enable_viscosity) {
+        why = "the compensated (Kahan) update is implemented for inviscid cases only";
+        return false;
+    }
+    return true;
+}
+
+int Engine::add_case(const fvm_params& p, int steps_total, bool all_sets, int* id_out) {
+    std::string why;
+    if (!params_valid(p, why)) return set_error(FVM_ERR_INVALID_ARGUMENT, why);
+    int rc = init_stream();
+    if (rc) return rc;
+    int id = -1;
+    for (int i = 0; i < (int)cases_.size(); i++)
+        if (!cases_[i]) { id = i; break; }
+    if (id < 0) {
+        id = (int)cases_.size();
+        if (id >= max_cases_) {
+            // grow the descriptor table (never while a graph references it)
+            int nmax = std::max(4, max_cases_ * 2);
+            while (nmax <= id) nmax *= 2;
+            void* nd = nullptr;
+            FVM_HIP(hipStreamSynchronize(stream_));
+            FVM_HIP(hipMalloc(&nd, desc_size_ * nmax));
+            if (d_cases_) {
+                FVM_HIP(hipMemcpy(nd, d_cases_, desc_size_ * max_cases_, hipMemcpyDeviceToDevice));
+                FVM_HIP(hipFree(d_cases_));
+            }
+            d_cases_ = nd;
+            h_descs_.resize(desc_size_ * nmax);
+            max_cases_ = nmax;
+            drop_graphs();
+        }
+        cases_.push_back(nullptr);
+    }
+    HostCase* hc = new HostCase();
+    hc->p = p;
+    hc->id = id;
+    hc->nsets = (all_sets || p.enable_viscosity) ? 4 : 3;
+    hc->steps_total = steps_total;
+    hc->tiles_x = (p.nx + tx_ - 1) / tx_;
+    hc->tiles_y = (p.ny + ty_ - 1) / ty_;
+    CaseLayout L = layout_for(p, fp64_, kahan_, hc->nsets);
+    hc->bytes = L.total;
+    if (use_arena_) {
+        hc->mem = arena_.alloc(L.total);
+        if (!hc->mem) {
+            delete hc;
+            cases_[id] = nullptr;
+            return set_error(FVM_ERR_CAPACITY, "device memory budget exhausted");
+        }
+    } else {
+        hipError_t e = hipMalloc(&hc->mem, L.total);
+        if (e != hipSuccess) {
+            delete hc;
+            cases_[id] = nullptr;
+            return hip_fail(e, "hipMalloc(case state)");
+        }
+    }
+    FVM_HIP(hipMemsetAsync(hc->mem, 0, L.total, stream_));
+    cases_[id] = hc;
+    rc = upload_desc(*hc);
+    if (rc) return rc;
+    hc->active = true;
+    rc = reset_case(id, p);
+    if (rc) return rc;
+    *id_out = id;
+    return FVM_OK;
+}
+
+int Engine::remove_case(int id) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    FVM_HIP(hipStreamSynchronize(stream_));
+    if (use_arena_) arena_.release(hc->mem);
+    else FVM_HIP(hipFree(hc->mem));
+    delete hc;
+    cases_[id] = nullptr;
+    lists_dirty_ = true;
+    if (!persistent_) drop_graphs();
+    return FVM_OK;
+}
+
+// Work list of one case's tiles in a scratch device buffer.
+static int upload_items(const std::vector<WorkItem>& v, WorkItem** buf, size_t* cap, int* d_count, int count_slot,
+                        hipStream_t s) {
+    if (v.size() > *cap) {
+        if (*buf) FVM_HIP(hipFree(*buf));
+        size_t nc = std::max<size_t>(v.size(), 64) * 2;
+        FVM_HIP(hipMalloc((void**)buf, nc * sizeof(WorkItem)));
+        *cap = nc;
+    }
+    if (!v.empty()) FVM_HIP(hipMemcpyAsync(*buf, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+    int n = (int)v.size();
+    FVM_HIP(hipMemcpyAsync(d_count + count_slot, &n, sizeof(int), hipMemcpyHostToDevice, s));
+    FVM_HIP(hipStreamSynchronize(s));
+    return FVM_OK;
+}
+
+// Tile order: consecutive dispatch slots go round-robin over the 8 XCDs, so a
+// contiguous block of tiles is handed to each XCD (halo rows stay in that
+// XCD's L2).  Holes are marked with index -1.
+static void xcd_order(const std::vector<WorkItem>& in, std::vector<WorkItem>& out) {
+    const int X = 8;
+    size_t n = in.size();
+    size_t chunk = (n + X - 1) / X;
+    out.assign(chunk * X, WorkItem{0, -1});
+    for (size_t t = 0; t < n; t++) {
+        size_t x = t / chunk, q = t % chunk;
+        out[q * X + x] = in[t];
+    }
+    while (!out.empty() && out.back().index < 0) out.pop_back();
+}
+
+int Engine::reset_case(int id, const fvm_params& p) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    std::string why;
+    if (!params_valid(p, why)) return set_error(FVM_ERR_INVALID_ARGUMENT, why);
+    if (p.nx != hc->p.nx || p.ny != hc->p.ny) return set_error(FVM_ERR_INVALID_ARGUMENT, "reset cannot change the grid size");
+    if (p.enable_viscosity && hc->nsets < 4) return set_error(FVM_ERR_INVALID_ARGUMENT, "case was admitted without the viscous buffer set");
+    hc->p = p;
+    int rc = upload_desc(*hc);
+    if (rc) return rc;
+    size_t n = (size_t)p.nx * p.ny;
+    int cur = current_set(*hc);
+    launch_init(fp64_, stream_, d_cases_, id, n, cur, 1);
+    launch_mark(fp64_, stream_, d_cases_, id, n, 0, 0);
+    FVM_HIP(hipGetLastError());
+    // time, steps, dt slots of this case
+    {
+        char* base = (char*)hc->mem;
+        CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+        FVM_HIP(hipMemsetAsync(base + L.slots_off, 0, 256 * 3, stream_));
+        if (kahan_) FVM_HIP(hipMemsetAsync(base + L.k_off, 0, round256(L.n * 4) * 5, stream_));
+    }
+    rc = mark_and_lists(*hc, 0);
+    if (rc) return rc;
+    hc->steps_done = 0;
+    hc->finished = false;
+    FVM_HIP(hipStreamSynchronize(stream_));
+    return FVM_OK;
+}
+
+// After a (re)mark: build the ghost list on the host, apply the BC of the
+// reset path (point-wise in place on the current set, ghost fill with the
+// pre-BC copy in set B) and the CFL maxima into the slot the next step reads.
+int Engine::mark_and_lists(HostCase& hc, int fixup) {
+    (void)fixup;
+    size_t n = (size_t)hc.p.nx * hc.p.ny;
+    std::vector<uint8_t> ct(n);
+    CaseLayout L = layout_for(hc.p, fp64_, kahan_, hc.nsets);
+    FVM_HIP(hipMemcpyAsync(ct.data(), (char*)hc.mem + L.ct_off, n, hipMemcpyDeviceToHost, stream_));
+    FVM_HIP(hipStreamSynchronize(stream_));
+    hc.ghosts.clear();
+    for (size_t i = 0; i < n; i++)
+        if (ct[i] == CT_GHOST) hc.ghosts.push_back((int)i);
+
+    int cur = current_set(hc);
+    std::vector<WorkItem> tiles, ordered, gl;
+    for (int t = 0; t < hc.tiles_x * hc.tiles_y; t++) tiles.push_back(WorkItem{hc.id, t});
+    xcd_order(tiles, ordered);
+    for (int g : hc.ghosts) gl.push_back(WorkItem{hc.id, g});
+    // per-case lists for the reset-path launches
+    WorkItem* d_t = nullptr;
+    WorkItem* d_g = nullptr;
+    size_t ct_cap = 0, cg_cap = 0;
+    int* d_cnt = nullptr;
+    FVM_HIP(hipMalloc((void**)&d_cnt, 4 * sizeof(int)));
+    int rc = upload_items(ordered, &d_t, &ct_cap, d_cnt, 0, stream_);
+    if (!rc) rc = upload_items(gl, &d_g, &cg_cap, d_cnt, 1, stream_);
+    if (rc) return rc;
+
+    Roles r{};
+    r.in = cur; r.out = cur; r.blend = -1; r.pre0 = 1; r.pre1 = -1;
+    r.slot_r = phase_; r.slot_w = phase_; r.zero_slot = 0; r.count_step = 0;
+    KernelCall kc{};
+    kc.stream = stream_;
+    kc.cases = d_cases_;
+    kc.work = d_t;
+    kc.count = d_cnt;
+    kc.ri = r;
+    kc.rv = r;
+    kc.grid = dim3((unsigned)std::max<size_t>(1, ordered.size()));
+    kc.op = OP_BC_INPLACE;
+    launch_tiled(fp64_, tx_, ty_, kc);
+    if (!gl.empty())
+        launch_ghost(fp64_, false, dim3((unsigned)((gl.size() + 255) / 256)), stream_, d_cases_, d_g, d_cnt + 1, r, r);
+    // CFL maxima of the reset state into the slot the next step reads
+    launch_zero_u32(stream_, (unsigned*)((char*)hc.mem + L.slots_off) + phase_ * 2, 2);
+    kc.op = OP_MAXIMA;
+    launch_tiled(fp64_, tx_, ty_, kc);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(stream_));
+    (void)hipFree(d_t);
+    if (d_g) (void)hipFree(d_g);
+    (void)hipFree(d_cnt);
+    lists_dirty_ = true;
+    if (!persistent_) drop_graphs();
+    return FVM_OK;
+}
+
+int Engine::update_params(int id, const fvm_params& p) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    std::string why;
+    if (!params_valid(p, why)) return set_error(FVM_ERR_INVALID_ARGUMENT, why);
+    if (p.nx != hc->p.nx || p.ny != hc->p.ny) return set_error(FVM_ERR_INVALID_ARGUMENT, "grid size changed; call resize/initialize");
+    if (p.enable_viscosity && hc->nsets < 4) return set_error(FVM_ERR_INVALID_ARGUMENT, "case has no viscous buffer set");
+    bool visc_changed = (p.enable_viscosity != 0) != (hc->p.enable_viscosity != 0);
+    if (visc_changed && !hc->p.enable_viscosity && phase_ == 1) {
+        // an inviscid case at odd parity holds U^n in set C; viscous cases keep it in A
+        size_t n = (size_t)p.nx * p.ny;
+        launch_copy_set(fp64_, stream_, d_cases_, id, n, 2, 0);
+    } else if (visc_changed && hc->p.enable_viscosity && phase_ == 1) {
+        size_t n = (size_t)p.nx * p.ny;
+        launch_copy_set(fp64_, stream_, d_cases_, id, n, 0, 2);
+    }
+    hc->p = p;
+    int rc = upload_desc(*hc);
+    if (rc) return rc;
+    if (visc_changed) {
+        lists_dirty_ = true;
+        if (!persistent_) drop_graphs();
+    }
+    return FVM_OK;
+}
+
+int Engine::update_obstacle(int id, const fvm_params& p) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    int rc = update_params(id, p);
+    if (rc) return rc;
+    size_t n = (size_t)p.nx * p.ny;
+    int cur = current_set(*hc);
+    launch_mark(fp64_, stream_, d_cases_, id, n, 1, cur);
+    launch_copy_set(fp64_, stream_, d_cases_, id, n, cur, 1);
+    FVM_HIP(hipGetLastError());
+    return mark_and_lists(*hc, 1);
+}
+
+int Engine::set_host_dt(int id, float dt) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    hc->p.dt = dt;
+    char* hd = h_descs_.data() + (size_t)id * desc_size_;
+    size_t off = fp64_ ? offsetof(CaseDesc<double>, c) + offsetof(CaseConsts<double>, host_dt)
+                       : offsetof(CaseDesc<float>, c) + offsetof(CaseConsts<float>, host_dt);
+    std::memcpy(hd + off, &dt, sizeof(float));
+    FVM_HIP(hipMemcpyAsync((char*)d_cases_ + (size_t)id * desc_size_ + off, hd + off, sizeof(float),
+                           hipMemcpyHostToDevice, stream_));
+    FVM_HIP(hipStreamSynchronize(stream_));
+    return FVM_OK;
+}
+
+int Engine::rebuild_lists() {
+    std::vector<WorkItem> all, visc, ghosts, oall, ovisc;
+    const bool had_visc = any_visc_;
+    WorkItem* const old_ptrs[3] = {d_tiles_all_, d_tiles_visc_, d_ghosts_};
+    any_visc_ = false;
+    for (HostCase* c : cases_) {
+        if (!c || !c->active || c->finished) continue;
+        int nt = c->tiles_x * c->tiles_y;
+        for (int t = 0; t < nt; t++) {
+            all.push_back(WorkItem{c->id, t});
+            if (c->p.enable_viscosity) visc.push_back(WorkItem{c->id, t});
+        }
+        if (c->p.enable_viscosity) any_visc_ = true;
+        for (int g : c->ghosts) ghosts.push_back(WorkItem{c->id, g});
+    }
+    xcd_order(all, oall);
+    xcd_order(visc, ovisc);
+    int rc = upload_items(oall, &d_tiles_all_, &cap_tiles_, d_counts_, 0, stream_);
+    if (!rc) rc = upload_items(ovisc, &d_tiles_visc_, &cap_visc_, d_counts_, 1, stream_);
+    if (!rc) rc = upload_items(ghosts, &d_ghosts_, &cap_ghosts_, d_counts_, 2, stream_);
+    if (rc) return rc;
+    n_tiles_ = (int)oall.size();
+    n_visc_ = (int)ovisc.size();
+    n_ghosts_ = (int)ghosts.size();
+    if (persistent_) {
+        // fixed grids: captured graphs read the lists from device memory and
+        // stay valid unless a list buffer moved or the viscous kernels appear
+        grid_tiles_ = persistent_grid_;
+        grid_visc_ = persistent_grid_;
+        grid_ghosts_ = 512;
+        if (had_visc != any_visc_ || old_ptrs[0] != d_tiles_all_ || old_ptrs[1] != d_tiles_visc_ ||
+            old_ptrs[2] != d_ghosts_)
+            drop_graphs();
+    } else {
+        grid_tiles_ = std::max(1, n_tiles_);
+        grid_visc_ = std::max(1, n_visc_);
+        grid_ghosts_ = std::max(1, (n_ghosts_ + 255) / 256);
+        drop_graphs();
+    }
+    lists_dirty_ = false;
+    return FVM_OK;
+}
+
+void Engine::drop_graphs() {
+    for (int i = 0; i < 2; i++)
+        if (graph_[i]) {
+            (void)hipGraphExecDestroy(graph_[i]);
+            graph_[i] = nullptr;
+        }
+    graph_valid_ = false;
+}
+
+int Engine::timed(const char* name, int rc_before) {
+    if (!timing_ || rc_before) return rc_before;
+    FVM_HIP(hipEventRecord(ev1_, stream_));
+    FVM_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    FVM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    for (auto& kv : timing_list_)
+        if (kv.first == name) {
+            kv.second.total_ms += ms;
+            kv.second.launches++;
+            return FVM_OK;
+        }
+    KernelStat st;
+    st.total_ms = ms;
+    st.launches = 1;
+    timing_list_.push_back({name, st});
+    return FVM_OK;
+}
+
+// One SSP-RK2 step of every active case, at step parity `ph`.
+int Engine::enqueue_step(int ph) {
+    const int cur = ph ? 2 : 0, nxt = ph ? 0 : 2;
+    const int sr = ph, sw = 1 - ph;
+    Roles i_s1{cur, 1, -1, -1, -1, sr, sw, 1, 1};
+    Roles i_g1{-1, 1, -1, cur, -1, sr, sw, 0, 0};
+    Roles i_s2{1, nxt, cur, -1, -1, sr, sw, 0, 0};
+    Roles i_g2{-1, nxt, -1, cur, 1, sr, sw, 0, 0};
+    Roles v_k0{0, 3, -1, -1, -1, sr, sw, 1, 1};
+    Roles v_s1{3, 1, -1, -1, -1, sr, sw, 0, 0};
+    Roles v_g1{-1, 1, -1, 3, -1, sr, sw, 0, 0};
+    Roles v_s2{1, 2, 3, -1, -1, sr, sw, 0, 0};
+    Roles v_k3{2, 0, -1, -1, -1, sr, sw, 0, 0};
+    Roles v_g2{-1, 0, -1, 2, -1, sr, sw, 0, 0};
+
+    KernelCall kc{};
+    kc.stream = stream_;
+    kc.cases = d_cases_;
+    kc.kahan = kahan_;
+    int rc = 0;
+    auto mark = [&]() {
+        if (timing_) (void)hipEventRecord(ev0_, stream_);
+    };
+    if (any_visc_) {
+        mark();
+        kc.op = OP_DIFFUSE;
+        kc.grid = dim3(grid_visc_);
+        kc.work = d_tiles_visc_;
+        kc.count = d_counts_ + 1;
+        kc.ri = v_k0;
+        kc.rv = v_k0;
+        launch_tiled(fp64_, tx_, ty_, kc);
+        rc = timed("diffuse_half1", rc);
+    }
+    mark();
+    kc.op = OP_STAGE1;
+    kc.grid = dim3(grid_tiles_);
+    kc.work = d_tiles_all_;
+    kc.count = d_counts_ + 0;
+    kc.ri = i_s1;
+    kc.rv = v_s1;
+    launch_tiled(fp64_, tx_, ty_, kc);
+    rc = timed("stage1", rc);
+    mark();
+    launch_ghost(fp64_, false, dim3(grid_ghosts_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g1, v_g1);
+    rc = timed("ghost1", rc);
+    mark();
+    kc.op = OP_STAGE2;
+    kc.ri = i_s2;
+    kc.rv = v_s2;
+    launch_tiled(fp64_, tx_, ty_, kc);
+    rc = timed("stage2", rc);
+    if (any_visc_) {
+        mark();
+        kc.op = OP_DIFFUSE_FINAL;
+        kc.grid = dim3(grid_visc_);
+        kc.work = d_tiles_visc_;
+        kc.count = d_counts_ + 1;
+        kc.ri = v_k3;
+        kc.rv = v_k3;
+        launch_tiled(fp64_, tx_, ty_, kc);
+        rc = timed("diffuse_half2", rc);
+    }
+    mark();
+    launch_ghost(fp64_, true, dim3(grid_ghosts_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g2, v_g2);
+    rc = timed("ghost2", rc);
+    if (rc) return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return FVM_OK;
+}
+
+int Engine::capture_graphs() {
+    drop_graphs();
+    for (int g = 0; g < 2; g++) {
+        hipGraph_t graph = nullptr;
+        FVM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(g);
+        if (!rc) rc = enqueue_step(1 - g);
+        hipError_t e = hipStreamEndCapture(stream_, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&graph_[g], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    }
+    graph_valid_ = true;
+    return FVM_OK;
+}
+
+void advance_counters(std::vector<HostCase*>& cases, int nsteps) {
+    for (HostCase* c : cases)
+        if (c && c->active && !c->finished) c->steps_done += nsteps;
+}
+
+int Engine::step_direct(int nsteps) {
+    if (lists_dirty_) {
+        int rc = rebuild_lists();
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        int rc = enqueue_step(phase_);
+        if (rc) return rc;
+        phase_ ^= 1;
+    }
+    advance_counters(cases_, nsteps);
+    return FVM_OK;
+}
+
+int Engine::step_graph(int nsteps) {
+    if (lists_dirty_) {
+        int rc = rebuild_lists();
+        if (rc) return rc;
+    }
+    if (timing_) return step_direct(nsteps);
+    int done = 0;
+    if (nsteps >= 2 && !graph_valid_) {
+        int rc = capture_graphs();
+        if (rc) return rc;
+    }
+    while (nsteps - done >= 2) {
+        FVM_HIP(hipGraphLaunch(graph_[phase_], stream_));
+        done += 2;
+    }
+    advance_counters(cases_, done);
+    if (done < nsteps) return step_direct(nsteps - done);
+    return FVM_OK;
+}
+
+int Engine::advance(int nsteps, bool use_graph) {
+    if (nsteps <= 0) return FVM_OK;
+    return use_graph ? step_graph(nsteps) : step_direct(nsteps);
+}
+
+int Engine::sync() {
+    FVM_HIP(hipStreamSynchronize(stream_));
+    return FVM_OK;
+}
+
+int Engine::read_state(int id, void* host, bool as_f64) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    FVM_HIP(hipStreamSynchronize(stream_));
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    int cur = current_set(*hc);
+    size_t es = fp64_ ? 8 : 4;
+    std::vector<char> tmp;
+    for (int f = 0; f < 5; f++) {
+        const char* src = (const char*)hc->mem + L.sets_off + L.plane * (cur * 5 + f);
+        if ((as_f64 && fp64_) || (!as_f64 && !fp64_)) {
+            FVM_HIP(hipMemcpy((char*)host + f * L.n * es, src, L.n * es, hipMemcpyDeviceToHost));
+        } else {
+            tmp.resize(L.n * es);
+            FVM_HIP(hipMemcpy(tmp.data(), src, L.n * es, hipMemcpyDeviceToHost));
+            if (as_f64) {
+                double* o = (double*)host + f * L.n;
+                const float* s = (const float*)tmp.data();
+                for (size_t i = 0; i < L.n; i++) o[i] = s[i];
+            } else {
+                float* o = (float*)host + f * L.n;
+                const double* s = (const double*)tmp.data();
+                for (size_t i = 0; i < L.n; i++) o[i] = (float)s[i];
+            }
+        }
+    }
+    return FVM_OK;
+}
+
+int Engine::write_state(int id, const void* host) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    FVM_HIP(hipStreamSynchronize(stream_));
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    int cur = current_set(*hc);
+    std::vector<double> tmp;
+    for (int f = 0; f < 5; f++) {
+        char* dst = (char*)hc->mem + L.sets_off + L.plane * (cur * 5 + f);
+        const float* s = (const float*)host + f * L.n;
+        if (fp64_) {
+            tmp.resize(L.n);
+            for (size_t i = 0; i < L.n; i++) tmp[i] = s[i];
+            FVM_HIP(hipMemcpy(dst, tmp.data(), L.n * 8, hipMemcpyHostToDevice));
+        } else {
+            FVM_HIP(hipMemcpy(dst, s, L.n * 4, hipMemcpyHostToDevice));
+        }
+    }
+    // maxima of the new state into the slot the next step reads
+    std::vector<WorkItem> tiles, ordered;
+    for (int t = 0; t < hc->tiles_x * hc->tiles_y; t++) tiles.push_back(WorkItem{hc->id, t});
+    xcd_order(tiles, ordered);
+    WorkItem* d_t = nullptr;
+    size_t cap = 0;
+    int* d_cnt = nullptr;
+    FVM_HIP(hipMalloc((void**)&d_cnt, 4 * sizeof(int)));
+    int rc = upload_items(ordered, &d_t, &cap, d_cnt, 0, stream_);
+    if (rc) return rc;
+    FVM_HIP(hipMemsetAsync((char*)hc->mem + L.slots_off + phase_ * 8, 0, 8, stream_));
+    if (kahan_) FVM_HIP(hipMemsetAsync((char*)hc->mem + L.k_off, 0, round256(L.n * 4) * 5, stream_));
+    Roles r{cur, cur, -1, -1, -1, phase_, phase_, 0, 0};
+    KernelCall kc{};
+    kc.op = OP_MAXIMA;
+    kc.stream = stream_;
+    kc.cases = d_cases_;
+    kc.work = d_t;
+    kc.count = d_cnt;
+    kc.ri = r;
+    kc.rv = r;
+    kc.grid = dim3((unsigned)std::max<size_t>(1, ordered.size()));
+    launch_tiled(fp64_, tx_, ty_, kc);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(stream_));
+    (void)hipFree(d_t);
+    (void)hipFree(d_cnt);
+    return FVM_OK;
+}
+
+int Engine::read_cell_types(int id, uint8_t* host) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    FVM_HIP(hipStreamSynchronize(stream_));
+    FVM_HIP(hipMemcpy(host, (char*)hc->mem + L.ct_off, L.n, hipMemcpyDeviceToHost));
+    return FVM_OK;
+}
+
+int Engine::read_sdf(int id, float* host) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    FVM_HIP(hipStreamSynchronize(stream_));
+    FVM_HIP(hipMemcpy(host, (char*)hc->mem + L.sdf_off, L.n * 4, hipMemcpyDeviceToHost));
+    return FVM_OK;
+}
+
+int Engine::read_time(int id, double* t, long long* steps) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    FVM_HIP(hipStreamSynchronize(stream_));
+    if (t) FVM_HIP(hipMemcpy(t, (char*)hc->mem + L.t_off, 8, hipMemcpyDeviceToHost));
+    if (steps) FVM_HIP(hipMemcpy(steps, (char*)hc->mem + L.steps_off, 8, hipMemcpyDeviceToHost));
+    return FVM_OK;
+}
+
+int Engine::queue_maxima(int id) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    FVM_HIP(hipMemcpyAsync(h_pinned_, (char*)hc->mem + L.slots_off + phase_ * 8, 8, hipMemcpyDeviceToHost, stream_));
+    return FVM_OK;
+}
+
+int Engine::read_maxima(int id, float* max_speed, float* max_nu) {
+    int rc = queue_maxima(id);
+    if (rc) return rc;
+    FVM_HIP(hipStreamSynchronize(stream_));
+    unsigned bits[2];
+    std::memcpy(bits, h_pinned_, 8);
+    float a, b;
+    std::memcpy(&a, &bits[0], 4);
+    std::memcpy(&b, &bits[1], 4);
+    if (max_speed) *max_speed = a;
+    if (max_nu) *max_nu = b;
+    return FVM_OK;
+}
+
+int Engine::field(int id, int fieldno, float* dev_out) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    if (fieldno < 0 || fieldno > 5) return set_error(FVM_ERR_INVALID_ARGUMENT, "unknown field");
+    launch_field(fp64_, stream_, d_cases_, id, (size_t)hc->p.nx * hc->p.ny, current_set(*hc), fieldno, dev_out);
+    FVM_HIP(hipGetLastError());
+    return FVM_OK;
+}
+
+int Engine::stat(int id, int mode, float* out) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    launch_zero_u32(stream_, d_stat_, 1);
+    launch_stat(fp64_, stream_, d_cases_, id, (size_t)hc->p.nx * hc->p.ny, current_set(*hc), mode, d_stat_);
+    FVM_HIP(hipGetLastError());
+    unsigned bits = 0;
+    FVM_HIP(hipMemcpyAsync(h_pinned_ + 4, d_stat_, 4, hipMemcpyDeviceToHost, stream_));
+    FVM_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(&bits, h_pinned_ + 4, 4);
+    std::memcpy(out, &bits, 4);
+    return FVM_OK;
+}
+
+int Engine::conservation(int id, double out[4]) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    launch_cons(fp64_, stream_, d_cases_, id, current_set(*hc), d_cons_ + 8, 1024, d_cons_);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(stream_));
+    FVM_HIP(hipMemcpy(out, d_cons_, 4 * sizeof(double), hipMemcpyDeviceToHost));
+    return FVM_OK;
+}
+
+}  // namespace fvm

@@ -1,0 +1,169 @@
+// fvm_engine.hpp -- host runtime: a set of cases resident on one GPU, their
+// work lists, the launch sequence of one step and its HIP-graph capture.
+// The plain solver (fvm_solver) is an engine with one case; the batched
+// engine (fvm_batch) holds hundreds.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fvm/fvm.h"
+#include "fvm_device.hpp"
+#include "fvm_kernels.hpp"
+
+namespace fvm {
+
+int set_error(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define FVM_HIP(call)                                  \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+// First-fit arena over one device allocation (256-byte granules).  Cases are
+// carved out of it so that admission and release never call hipMalloc/hipFree
+// while other cases are running.
+class Arena {
+public:
+    ~Arena();
+    int init(size_t bytes);
+    void* alloc(size_t bytes);
+    void release(void* p);
+    size_t capacity() const { return cap_; }
+    size_t in_use() const { return used_; }
+    bool ready() const { return base_ != nullptr; }
+
+private:
+    char* base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+    std::map<size_t, size_t> live_;  // offset -> size
+};
+
+struct HostCase {
+    fvm_params p{};
+    int id = -1;
+    bool active = false;
+    int nsets = 0;
+    size_t bytes = 0;
+    void* mem = nullptr;
+    std::vector<int> ghosts;
+    int tiles_x = 0, tiles_y = 0;
+    int steps_total = 0;   // batch: steps to run before completion (0 = unbounded)
+    long long steps_done = 0;
+    bool finished = false;
+};
+
+struct KernelStat {
+    double total_ms = 0;
+    long long launches = 0;
+};
+
+class Engine {
+public:
+    Engine(int device, bool fp64, bool kahan);
+    ~Engine();
+
+    int init_stream();
+    int set_tile(int tx, int ty);
+    int tile_x() const { return tx_; }
+    int tile_y() const { return ty_; }
+
+    // case lifetime
+    static size_t case_bytes(const fvm_params& p, bool fp64, bool kahan, bool all_sets);
+    int add_case(const fvm_params& p, int steps_total, bool all_sets, int* id);
+    int remove_case(int id);
+    int reset_case(int id, const fvm_params& p);
+    int update_params(int id, const fvm_params& p);   // constants only, same geometry
+    int update_obstacle(int id, const fvm_params& p);
+    int set_host_dt(int id, float dt);
+
+    // stepping
+    int step_direct(int nsteps);        // launches without a graph
+    int step_graph(int nsteps);         // replays a captured 2-step graph
+    int advance(int nsteps, bool use_graph);
+    int sync();
+
+    // per-case access
+    HostCase* get(int id);
+    int current_set(const HostCase& c) const;   // set holding U^n
+    int latest_slot() const { return phase_; }  // slot holding the maxima of the current state
+    int read_state(int id, void* host, bool as_f64);
+    int write_state(int id, const void* host);
+    int read_cell_types(int id, uint8_t* host);
+    int read_sdf(int id, float* host);
+    int read_time(int id, double* t, long long* steps);
+    int read_maxima(int id, float* max_speed, float* max_nu);
+    int queue_maxima(int id);
+    int field(int id, int field, float* dev_out);
+    int stat(int id, int mode, float* out);
+    int conservation(int id, double out[4]);
+
+    int active_count() const;
+    int case_slots() const { return (int)cases_.size(); }
+    void mark_lists_dirty() { lists_dirty_ = true; }
+    int retire_finished();
+    size_t bytes_in_use() const;
+    int set_arena(size_t bytes);
+    bool fp64() const { return fp64_; }
+    bool kahan() const { return kahan_; }
+    hipStream_t stream() const { return stream_; }
+
+    void enable_timing(bool on) { timing_ = on; }
+    const std::vector<std::pair<std::string, KernelStat>>& timings() const { return timing_list_; }
+    void clear_timings() { timing_list_.clear(); }
+
+private:
+    int build_consts(HostCase& hc, const fvm_params& p, void* desc_out);
+    int upload_desc(HostCase& hc);
+    int mark_and_lists(HostCase& hc, int fixup);
+    int rebuild_lists();
+    int enqueue_step(int phase);
+    int capture_graphs();
+    void drop_graphs();
+    int timed(const char* name, int rc_before);
+
+    int device_;
+    bool fp64_, kahan_;
+    int tx_ = 64, ty_ = 8;
+    hipStream_t stream_ = nullptr;
+    std::vector<HostCase*> cases_;
+    int max_cases_ = 0;
+    void* d_cases_ = nullptr;        // CaseDesc<T>[max_cases_]
+    size_t desc_size_ = 0;
+    std::vector<char> h_descs_;      // host mirror
+
+    WorkItem* d_tiles_all_ = nullptr;
+    WorkItem* d_tiles_visc_ = nullptr;
+    WorkItem* d_ghosts_ = nullptr;
+    int* d_counts_ = nullptr;        // [0] all tiles, [1] viscous tiles, [2] ghosts
+    size_t cap_tiles_ = 0, cap_visc_ = 0, cap_ghosts_ = 0;
+    int n_tiles_ = 0, n_visc_ = 0, n_ghosts_ = 0;
+    int grid_tiles_ = 0, grid_visc_ = 0, grid_ghosts_ = 0;
+    bool any_visc_ = false;
+    bool persistent_ = false;
+    int persistent_grid_ = 2048;
+
+    int phase_ = 0;                  // step parity: inviscid current set and dt slot
+    bool lists_dirty_ = true;
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    bool graph_valid_ = false;
+
+    Arena arena_;
+    bool use_arena_ = false;
+
+    bool timing_ = false;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    std::vector<std::pair<std::string, KernelStat>> timing_list_;
+
+    unsigned* d_stat_ = nullptr;
+    double* d_cons_ = nullptr;
+    float* h_pinned_ = nullptr;
+};
+
+}  // namespace fvm

@@ -1,0 +1,926 @@
+// fvm_kernels.hip -- hand-written gfx950 kernels of the FVM time step.
+//
+// Kernels of one SSP-RK2 step (src/solver.cu:3289-3412), all over a work list
+// of (case, tile) items so that one launch serves one case or hundreds:
+//   k_diffuse<final=false>   viscous half step D(dt/2) on U^n  -> U_d        (viscous only)
+//   k_stage<STAGE1>          prims + MUSCL + Riemann flux + FV update + point-wise BC -> U*
+//   k_ghost                  immersed-boundary ghost fill of U*
+//   k_stage<STAGE2>          second stage + SSP blend (+ point-wise BC + CFL maxima when inviscid)
+//   k_diffuse<final=true>    second half step + point-wise BC + CFL maxima (viscous only)
+//   k_ghost<final>           ghost fill + CFL maxima of the ghost cells
+// Each stage kernel stages its tile plus a 2-cell halo through LDS, derives
+// the primitives once per cell, computes every face flux once per tile and
+// keeps flux divergence in registers; no face array ever reaches HBM.
+#include "fvm_device.hpp"
+#include "fvm_kernels.hpp"
+
+#include <algorithm>
+
+namespace fvm {
+
+constexpr int NT = 256;
+
+// ---------------------------------------------------------------------------
+// Obstacle marking (computeSDFKernel, src/solver.cu:536-589).  IEEE division
+// and square root, no contraction: the mask must equal the CPU oracle's bit for
+// bit.  Rotation and star vertices arrive precomputed from the host.
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+__device__ float dist_to_segment(float px, float py, float ax, float ay, float bx, float by) {
+    float abx = bx - ax, aby = by - ay;
+    float apx = px - ax, apy = py - ay;
+    float t = __fdiv_rn(apx * abx + apy * aby, abx * abx + aby * aby + 1e-10f);
+    t = fmaxf(0.0f, fminf(1.0f, t));
+    float cx = ax + t * abx, cy = ay + t * aby;
+    float dx = px - cx, dy = py - cy;
+    return __fsqrt_rn(dx * dx + dy * dy);
+}
+
+__device__ __forceinline__ float cross2d(float ax, float ay, float bx, float by) { return ax * by - ay * bx; }
+
+__device__ float sdf_quad(float px, float py, float v0x, float v0y, float v1x, float v1y,
+                          float v2x, float v2y, float v3x, float v3y) {
+    float d0 = dist_to_segment(px, py, v0x, v0y, v1x, v1y);
+    float d1 = dist_to_segment(px, py, v1x, v1y, v2x, v2y);
+    float d2 = dist_to_segment(px, py, v2x, v2y, v3x, v3y);
+    float d3 = dist_to_segment(px, py, v3x, v3y, v0x, v0y);
+    float m = fminf(fminf(d0, d1), fminf(d2, d3));
+    float c0 = cross2d(v1x - v0x, v1y - v0y, px - v0x, py - v0y);
+    float c1 = cross2d(v2x - v1x, v2y - v1y, px - v1x, py - v1y);
+    float c2 = cross2d(v3x - v2x, v3y - v2y, px - v2x, py - v2y);
+    float c3 = cross2d(v0x - v3x, v0y - v3y, px - v3x, py - v3y);
+    bool inside = (c0 >= 0 && c1 >= 0 && c2 >= 0 && c3 >= 0) || (c0 <= 0 && c1 <= 0 && c2 <= 0 && c3 <= 0);
+    return inside ? -m : m;
+}
+
+__device__ float shape_sdf(const Geom& k, float px, float py) {
+    const float cx = k.obs_x, cy = k.obs_y, r = k.obs_r;
+    float lx = px - cx, ly = py - cy;
+    float qx = lx * k.cosR - ly * k.sinR;
+    float qy = lx * k.sinR + ly * k.cosR;
+    switch (k.shape) {
+    default:
+    case 0:  // circle
+        return __fsqrt_rn((px - cx) * (px - cx) + (py - cy) * (py - cy)) - r;
+    case 1: {  // five-point star: distance to the 10 edges, winding-number sign
+        float minDist = 1e10f, winding = 0.0f;
+        for (int i = 0; i < 10; i++) {
+            int j = (i + 1) % 10;
+            float ax = k.star[i * 2], ay = k.star[i * 2 + 1];
+            float bx = k.star[j * 2], by = k.star[j * 2 + 1];
+            minDist = fminf(minDist, dist_to_segment(qx, qy, ax, ay, bx, by));
+            float eax = ax - qx, eay = ay - qy, ebx = bx - qx, eby = by - qy;
+            winding += atan2f(cross2d(eax, eay, ebx, eby), eax * ebx + eay * eby);
+        }
+        float sign = (fabsf(winding) > 3.14159265358979323846f) ? -1.0f : 1.0f;
+        return sign * minDist;
+    }
+    case 2: {  // diamond
+        float ndx = __fdiv_rn(fabsf(qx), r), ndy = __fdiv_rn(fabsf(qy), r);
+        return (ndx + ndy - 1.0f) * r * 0.7071f;
+    }
+    case 3: {  // capsule
+        float hw = r * 1.5f, cr = r * 0.5f;
+        float clampedX = fmaxf(-hw + cr, fminf(hw - cr, qx));
+        float dx = qx - clampedX, dy = qy;
+        return __fsqrt_rn(dx * dx + dy * dy) - cr;
+    }
+    case 4: {  // triangle pointing along +x
+        const float s32 = 0.866025404f;
+        float v0x = r, v0y = 0.0f, v1x = -r * 0.5f, v1y = r * s32, v2x = -r * 0.5f, v2y = -r * s32;
+        float d0 = dist_to_segment(qx, qy, v0x, v0y, v1x, v1y);
+        float d1 = dist_to_segment(qx, qy, v1x, v1y, v2x, v2y);
+        float d2 = dist_to_segment(qx, qy, v2x, v2y, v0x, v0y);
+        float m = fminf(d0, fminf(d1, d2));
+        float c0 = cross2d(v1x - v0x, v1y - v0y, qx - v0x, qy - v0y);
+        float c1 = cross2d(v2x - v1x, v2y - v1y, qx - v1x, qy - v1y);
+        float c2 = cross2d(v0x - v2x, v0y - v2y, qx - v2x, qy - v2y);
+        bool inside = (c0 >= 0 && c1 >= 0 && c2 >= 0) || (c0 <= 0 && c1 <= 0 && c2 <= 0);
+        return inside ? -m : m;
+    }
+    case 5: {  // starship: circle body, fixed wing root, rotated wing (common.h:28-36)
+        float distToCircle = __fsqrt_rn(qx * qx + qy * qy) - r;
+        float scale = __fdiv_rn(r, 90.0f);
+        float refX = __fdiv_rn(qx, scale);
+        float refY = __fdiv_rn(fabsf(qy), scale);
+        float dRoot = sdf_quad(refX, refY, -45.0f, 77.9422f, 10.9682f, 89.329f, 6.0f, 112.0f, -6.0f, 112.0f) * scale;
+        float pX = -6.0f, pY = 112.0f;
+        float dx1 = 6.0f - pX, dy1 = 112.0f - pY;
+        float w1x = pX + dx1 * k.cosW - dy1 * k.sinW, w1y = pY + dx1 * k.sinW + dy1 * k.cosW;
+        float dx2 = 3.0f - pX, dy2 = 186.0f - pY;
+        float w2x = pX + dx2 * k.cosW - dy2 * k.sinW, w2y = pY + dx2 * k.sinW + dy2 * k.cosW;
+        float dx3 = -3.0f - pX, dy3 = 186.0f - pY;
+        float w3x = pX + dx3 * k.cosW - dy3 * k.sinW, w3y = pY + dx3 * k.sinW + dy3 * k.cosW;
+        float dWing = sdf_quad(refX, refY, pX, pY, w1x, w1y, w2x, w2y, w3x, w3y) * scale;
+        return fminf(distToCircle, fminf(dRoot, dWing));
+    }
+    }
+}
+
+__device__ __forceinline__ uint8_t classify(float d, int i, int nx, float band) {
+    uint8_t t;
+    if (d < -band) t = CT_SOLID;
+    else if (d < band && d >= -band) t = CT_GHOST;
+    else t = CT_FLUID;
+    if (i == 0) t = CT_INFLOW;
+    else if (i == nx - 1) t = CT_OUTFLOW;
+    return t;
+}
+
+// Marks one case; with fixup != 0 it is updateSDFWithFixupKernel
+// (solver.cu:607-664): cells that turn from solid/ghost into fluid are zeroed
+// in buffer set `fix_set`.
+template <class T>
+__global__ __launch_bounds__(NT) void k_mark(const CaseDesc<T>* __restrict__ cases, int case_id, int fixup, int fix_set) {
+    const CaseDesc<T>& cd = cases[case_id];
+    const CaseConsts<T>& k = cd.c;
+    size_t n = (size_t)k.nx * k.ny;
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (size_t)gridDim.x * NT) {
+        int i = (int)(idx % k.nx), j = (int)(idx / k.nx);
+        float x = ((float)i + 0.5f) * k.geo.dx;
+        float y = ((float)j + 0.5f) * k.geo.dy;
+        float d = shape_sdf(k.geo, x, y);
+        uint8_t old = cd.b.ct[idx];
+        uint8_t t = classify(d, i, k.nx, k.geo.band);
+        cd.b.sdf[idx] = d;
+        cd.b.ct[idx] = t;
+        if (fixup && (old == CT_SOLID || old == CT_GHOST) && t == CT_FLUID) {
+            for (int f = 0; f < 5; f++) ((T*)cd.b.U[fix_set][f])[idx] = T(0);
+        }
+    }
+}
+
+// Seeded counter-based perturbation: splitmix64 finaliser of
+// (seed * golden + idx + 1); top 24 bits -> u in [0,1); eps = amp (2u - 1).
+__device__ __forceinline__ float noise_unit_f(uint32_t seed, unsigned long long idx) {
+    unsigned long long z = (unsigned long long)seed * 0x9E3779B97F4A7C15ull + idx + 1ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (float)(z >> 40) * (1.0f / 16777216.0f);
+}
+
+// initializeKernel (src/solver.cu:911-940) + perturbation; writes sets a and b.
+template <class T>
+__global__ __launch_bounds__(NT) void k_init(const CaseDesc<T>* __restrict__ cases, int case_id, int set_a, int set_b) {
+    const CaseDesc<T>& cd = cases[case_id];
+    const CaseConsts<T>& k = cd.c;
+    size_t n = (size_t)k.nx * k.ny;
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (size_t)gridDim.x * NT) {
+        T rho = (T)k.rho_inf, u = (T)k.u_inf, v = (T)k.v_inf;
+        if (k.noise_field != 0) {
+            T eps = (T)k.noise_amp * (T)(2.0f * noise_unit_f(k.noise_seed, idx) - 1.0f);
+            if (k.noise_field == 1) rho = (T)k.rho_inf * (T(1) + eps);
+            else v = (T)k.v_inf + (T)k.u_inf * eps;
+        }
+        T ke = T(0.5) * rho * (u * u + v * v);
+        T e = (T)k.p_inf / (T)k.gm1;
+        T vals[5] = {rho, rho * u, rho * v, e + ke, e};
+        for (int f = 0; f < 5; f++) {
+            ((T*)cd.b.U[set_a][f])[idx] = vals[f];
+            if (set_b >= 0) ((T*)cd.b.U[set_b][f])[idx] = vals[f];
+        }
+    }
+}
+#pragma clang fp contract(on)
+
+// Copy buffer set `src` to `dst` for one case.
+template <class T>
+__global__ __launch_bounds__(NT) void k_copy_set(const CaseDesc<T>* __restrict__ cases, int case_id, int src, int dst) {
+    const CaseDesc<T>& cd = cases[case_id];
+    size_t n = (size_t)cd.c.nx * cd.c.ny;
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < n; idx += (size_t)gridDim.x * NT)
+        for (int f = 0; f < 5; f++) ((T*)cd.b.U[dst][f])[idx] = ((const T*)cd.b.U[src][f])[idx];
+}
+
+// Point-wise BC in place on buffer set r.out over a tile list (reset path).
+template <class T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_bc_inplace(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
+                                                   const int* __restrict__ count, Roles r) {
+    const int n_items = *count;
+    for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
+        WorkItem it = work[w];
+        if (it.index < 0) continue;
+        const CaseDesc<T>& cd = cases[it.case_id];
+        const CaseConsts<T>& k = cd.c;
+        const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
+        for (int q = threadIdx.x; q < TX * TY; q += NT) {
+            int gi = i0 + q % TX, gj = j0 + q / TX;
+            if (gi >= k.nx || gj >= k.ny) continue;
+            size_t g = (size_t)gj * k.nx + gi;
+            uint8_t c = cd.b.ct[g];
+            if (c == CT_GHOST) continue;
+            T v[5];
+            for (int f = 0; f < 5; f++) v[f] = ((T*)cd.b.U[r.out][f])[g];
+            if (bc_pointwise(k, gi, gj, c, v))
+                for (int f = 0; f < 5; f++) ((T*)cd.b.U[r.out][f])[g] = v[f];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused RK stage.  MODE 0: stage 1 (update + point-wise BC).  MODE 1: stage 2
+// of an inviscid case (update + blend + point-wise BC + CFL maxima) or of a
+// viscous case (update + blend only, no BC: the second half step follows).
+// KAHAN: the final blend is written as U^n + delta with a carried compensation.
+// ri / rv: roles for inviscid / viscous cases.
+// ---------------------------------------------------------------------------
+template <class T, int TX, int TY, int MODE, bool KAHAN>
+__global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
+                                              const int* __restrict__ count, Roles ri, Roles rv) {
+    constexpr int SX = TX + 4, SY = TY + 4, CPT = (TX * TY) / NT;
+    static_assert(CPT * NT == TX * TY, "tile must be a multiple of the workgroup size");
+    constexpr int FX = (TX + 1) * TY, FY = TX * (TY + 1);
+    constexpr int FMAX = FX > FY ? FX : FY;
+    __shared__ T s_r[SY * SX], s_u[SY * SX], s_v[SY * SX], s_p[SY * SX];
+    __shared__ uint8_t s_ct[SY * SX];
+    __shared__ T s_f[5][FMAX];
+    __shared__ float s_red[NT / 64];
+
+    const int n_items = *count;
+    for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
+        const WorkItem it = work[w];
+        if (it.index < 0) continue;
+        const CaseDesc<T>& cd = cases[it.case_id];
+        const CaseConsts<T>& k = cd.c;
+        const bool visc = k.visc != 0;
+        const Roles r = visc ? rv : ri;
+        const int nx = k.nx, ny = k.ny;
+        const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
+        const float dtf = dt_from_slot(k, cd.b.slots, r.slot_r);
+        const T dt = (T)dtf;
+
+        if (it.index == 0 && threadIdx.x == 0) {
+            if (r.zero_slot) { cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u; }
+            if (r.count_step) { cd.b.t[0] += (double)dtf; cd.b.steps[0] += 1; }
+        }
+
+        const T* __restrict__ in0 = (const T*)cd.b.U[r.in][0];
+        const T* __restrict__ in1 = (const T*)cd.b.U[r.in][1];
+        const T* __restrict__ in2 = (const T*)cd.b.U[r.in][2];
+        const T* __restrict__ in3 = (const T*)cd.b.U[r.in][3];
+        const T* __restrict__ in4 = (const T*)cd.b.U[r.in][4];
+        const uint8_t* __restrict__ ctp = cd.b.ct;
+
+        // ---- stage tile + halo 2: conserved -> (rho, u, v, p) in LDS
+        for (int n = threadIdx.x; n < SX * SY; n += NT) {
+            int li = n % SX, lj = n / SX;
+            int gi = min(max(i0 + li - 2, 0), nx - 1);
+            int gj = min(max(j0 + lj - 2, 0), ny - 1);
+            size_t g = (size_t)gj * nx + gi;
+            T rho = in0[g];
+            T u, v, p;
+            primitives_uvp(k, rho, in1[g], in2[g], in3[g], in4[g], u, v, p);
+            s_r[n] = rho; s_u[n] = u; s_v[n] = v; s_p[n] = p; s_ct[n] = ctp[g];
+        }
+        __syncthreads();
+
+        // ---- x faces i+1/2 for local i in [-1, TX-1]
+        for (int n = threadIdx.x; n < FX; n += NT) {
+            int fi = n % (TX + 1) - 1, fj = n / (TX + 1);
+            int gi = i0 + fi, gj = j0 + fj;
+            T o[5] = {T(0), T(0), T(0), T(0), T(0)};
+            if (gi >= 0 && gi < nx - 1 && gj < ny) {
+                int b = (fj + 2) * SX + (fi + 1);
+                T rr[4], un[4], ut[4], pp[4];
+                uint8_t cc[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    rr[m] = s_r[b + m]; un[m] = s_u[b + m]; ut[m] = s_v[b + m]; pp[m] = s_p[b + m]; cc[m] = s_ct[b + m];
+                }
+                face_flux(k, rr, un, ut, pp, cc, o);
+            }
+#pragma unroll
+            for (int c = 0; c < 5; c++) s_f[c][n] = o[c];
+        }
+        __syncthreads();
+
+        T res[CPT][5];
+        T fxr[CPT], fxl[CPT], fyt[CPT], fyb[CPT];
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            int n = threadIdx.x + q * NT;
+            int li = n % TX, lj = n / TX;
+            int a = lj * (TX + 1) + li;
+#pragma unroll
+            for (int c = 0; c < 5; c++) res[q][c] = (s_f[c][a + 1] - s_f[c][a]) * (T)k.inv_dx;
+            fxr[q] = s_f[0][a + 1];
+            fxl[q] = s_f[0][a];
+        }
+        __syncthreads();
+
+        // ---- y faces j+1/2 for local j in [-1, TY-1]
+        for (int n = threadIdx.x; n < FY; n += NT) {
+            int fi = n % TX, fj = n / TX - 1;
+            int gi = i0 + fi, gj = j0 + fj;
+            T o[5] = {T(0), T(0), T(0), T(0), T(0)};
+            if (gj >= 0 && gj < ny - 1 && gi < nx) {
+                int b = (fj + 1) * SX + (fi + 2);
+                T rr[4], un[4], ut[4], pp[4];
+                uint8_t cc[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    int s = b + m * SX;
+                    rr[m] = s_r[s]; un[m] = s_v[s]; ut[m] = s_u[s]; pp[m] = s_p[s]; cc[m] = s_ct[s];
+                }
+                face_flux(k, rr, un, ut, pp, cc, o);
+            }
+            s_f[0][n] = o[0]; s_f[1][n] = o[2]; s_f[2][n] = o[1]; s_f[3][n] = o[3]; s_f[4][n] = o[4];
+        }
+        __syncthreads();
+
+        float vmax_local = 0.0f;
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            int n = threadIdx.x + q * NT;
+            int li = n % TX, lj = n / TX;
+            int gi = i0 + li, gj = j0 + lj;
+            int a = lj * TX + li;
+            T dfy[5];
+#pragma unroll
+            for (int c = 0; c < 5; c++) dfy[c] = (s_f[c][a + TX] - s_f[c][a]) * (T)k.inv_dy;
+            fyt[q] = s_f[0][a + TX];
+            fyb[q] = s_f[0][a];
+            if (gi >= nx || gj >= ny) continue;
+            const size_t g = (size_t)gj * nx + gi;
+            const int sc = (lj + 2) * SX + (li + 2);
+            const uint8_t c = s_ct[sc];
+            T v0[5] = {s_r[sc], in1[g], in2[g], in3[g], in4[g]};
+            T nv[5];
+            if (c == CT_SOLID || c == CT_GHOST) {
+#pragma unroll
+                for (int f = 0; f < 5; f++) nv[f] = v0[f];
+            } else {
+                // updateKernel, src/solver.cu:1736-1868
+                const bool ex = (gi == 0 || gi == nx - 1), ey = (gj == 0 || gj == ny - 1);
+                T src = T(0);
+                if (!ex && !ey) {
+                    T rc = tmax(s_r[sc], (T)k.min_rho);
+                    T rxR = tmax(s_r[sc + 1], (T)k.min_rho), rxL = tmax(s_r[sc - 1], (T)k.min_rho);
+                    T ryT = tmax(s_r[sc + SX], (T)k.min_rho), ryB = tmax(s_r[sc - SX], (T)k.min_rho);
+                    T ufR = T(2) * fxr[q] * Num<T>::rcp(rc + rxR);
+                    T ufL = T(2) * fxl[q] * Num<T>::rcp(rxL + rc);
+                    T vfT = T(2) * fyt[q] * Num<T>::rcp(rc + ryT);
+                    T vfB = T(2) * fyb[q] * Num<T>::rcp(ryB + rc);
+                    T divv = (ufR - ufL) * (T)k.inv_dx + (vfT - vfB) * (T)k.inv_dy;
+                    src = -s_p[sc] * divv;
+                }
+                T d[5];
+#pragma unroll
+                for (int f = 0; f < 5; f++) d[f] = (ex ? T(0) : res[q][f]) + (ey ? T(0) : dfy[f]);
+                T nr = v0[0] - dt * d[0];
+                T nru = v0[1] - dt * d[1];
+                T nrv = v0[2] - dt * d[2];
+                T nE = v0[3] - dt * d[3];
+                T nre = v0[4] - dt * d[4] + dt * src;
+                if (!Num<T>::finite(nr) || nr < (T)k.min_rho) nr = tmax(v0[0], (T)k.min_rho);
+                if (!Num<T>::finite(nru)) nru = v0[1];
+                if (!Num<T>::finite(nrv)) nrv = v0[2];
+                if (!Num<T>::finite(nE)) nE = v0[3];
+                if (!Num<T>::finite(nre) || nre < (T)k.min_p * (T)k.inv_gm1) nre = v0[4];
+                nr = tmax(nr, (T)k.min_rho);
+                T ir = Num<T>::rcp(nr);
+                T un = nru * ir, vn = nrv * ir;
+                T vm = Num<T>::sqrt(un * un + vn * vn);
+                if (vm > (T)k.vmax) {
+                    T s = (T)k.vmax * Num<T>::rcp(vm);
+                    un *= s; vn *= s;
+                    nru = nr * un; nrv = nr * vn;
+                }
+                T ke = T(0.5) * nr * (un * un + vn * vn);
+                T eta = (nE > T(0)) ? Num<T>::div(nre, nE) : T(1);
+                T e = (eta < T(0.001f)) ? nre : nE - ke;
+                T rRi = nr * (T)k.R * (T)k.inv_gm1;
+                e = tmax(e, rRi * (T)k.min_T);
+                e = tmin(e, rRi * (T)k.max_T);
+                nv[0] = nr; nv[1] = nru; nv[2] = nrv; nv[3] = e + ke; nv[4] = e;
+            }
+            T out[5];
+            float* kc[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+            T carry[5] = {T(0), T(0), T(0), T(0), T(0)};
+            if (MODE == 0) {
+#pragma unroll
+                for (int f = 0; f < 5; f++) out[f] = nv[f];
+            } else {
+#pragma unroll
+                for (int f = 0; f < 5; f++) {
+                    T bl = ((const T*)cd.b.U[r.blend][f])[g];
+                    if (KAHAN) {
+                        kc[f] = cd.b.K[f];
+                        T delta = T(0.5) * (nv[f] - bl);
+                        T y = delta - (T)kc[f][g];
+                        T t = bl + y;
+                        carry[f] = (t - bl) - y;
+                        out[f] = t;
+                    } else {
+                        out[f] = T(0.5) * bl + T(0.5) * nv[f];
+                    }
+                }
+            }
+            const bool apply_bc = (MODE == 0) || !visc;
+            if (apply_bc) {
+                if (c == CT_GHOST) {
+                    if (KAHAN && MODE == 1)
+                        for (int f = 0; f < 5; f++) kc[f][g] = 0.0f;
+                    continue;  // filled by k_ghost
+                }
+                bool touched = bc_pointwise(k, gi, gj, c, out);
+                if (MODE == 1) vmax_local = fmaxf(vmax_local, (float)wave_speed(k, out));
+                if (KAHAN && MODE == 1)
+                    for (int f = 0; f < 5; f++) kc[f][g] = touched ? 0.0f : (float)carry[f];
+            }
+#pragma unroll
+            for (int f = 0; f < 5; f++) ((T*)cd.b.U[r.out][f])[g] = out[f];
+        }
+        if (MODE == 1 && !visc) block_max_atomic<NT>(vmax_local, s_red, &cd.b.slots[r.slot_w * 2]);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Immersed-boundary ghost fill (applyBoundaryConditionsKernel ghost branch,
+// src/solver.cu:2045-2225) over a list of ghost cells.  A neighbour that is a
+// ghost cell is read at its pre-BC value (set pre0, or the SSP blend of pre0
+// and pre1); any other neighbour at its final value in set out.  Normal and
+// neighbour choice use IEEE arithmetic so the choice matches the oracle.
+// FINAL: also the CFL maxima of the filled cells.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T pre_val(const CaseBufs& b, const Roles& r, size_t n, int f) {
+    T a = ((const T*)b.U[r.pre0][f])[n];
+    if (r.pre1 < 0) return a;
+    T c = ((const T*)b.U[r.pre1][f])[n];
+    return T(0.5) * a + T(0.5) * c;
+}
+
+#pragma clang fp contract(off)
+template <class T, bool FINAL>
+__global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ items,
+                                              const int* __restrict__ count, Roles ri, Roles rv) {
+    const int n_items = *count;
+    for (int w = blockIdx.x * NT + threadIdx.x; w < n_items; w += gridDim.x * NT) {
+        const WorkItem it = items[w];
+        const CaseDesc<T>& cd = cases[it.case_id];
+        const CaseConsts<T>& k = cd.c;
+        const Roles r = k.visc ? rv : ri;
+        const CaseBufs& b = cd.b;
+        const int nx = k.nx, ny = k.ny;
+        const size_t idx = (size_t)it.index;
+        const int i = (int)(idx % nx), j = (int)(idx / nx);
+        const float* sdf = b.sdf;
+        const uint8_t* ct = b.ct;
+        const T* out0 = (const T*)b.U[r.out][0];
+        float sxp = (i < nx - 1) ? sdf[idx + 1] : sdf[idx];
+        float sxm = (i > 0) ? sdf[idx - 1] : sdf[idx];
+        float syp = (j < ny - 1) ? sdf[idx + nx] : sdf[idx];
+        float sym = (j > 0) ? sdf[idx - nx] : sdf[idx];
+        T nxs = (T)((T)sxp - (T)sxm) / ((T)2 * (T)k.dx);
+        T nys = (T)((T)syp - (T)sym) / ((T)2 * (T)k.dy);
+        T nl = ::sqrt(nxs * nxs + nys * nys) + (T)1e-10f;
+        nxs = nxs / nl;
+        nys = nys / nl;
+        const int ox[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+        const int oy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+        long long best = -1;
+        T best_dot = T(-1);
+        for (int m = 0; m < 8; m++) {
+            int ni = i + ox[m], nj = j + oy[m];
+            if (ni < 0 || ni >= nx || nj < 0 || nj >= ny) continue;
+            size_t nidx = (size_t)nj * nx + ni;
+            if (!(sdf[nidx] > 0.0f)) continue;
+            T rn = (ct[nidx] == CT_GHOST) ? pre_val<T>(b, r, nidx, 0) : out0[nidx];
+            if (!(rn >= (T)k.min_rho)) continue;
+            T dxn = (T)ox[m], dyn = (T)oy[m];
+            T len = ::sqrt(dxn * dxn + dyn * dyn);
+            T dot = (dxn * nxs + dyn * nys) / len;
+            if (dot > best_dot) { best_dot = dot; best = (long long)nidx; }
+        }
+        T o[5];
+        if (best < 0) {
+            o[0] = (T)k.rho_inf; o[1] = T(0); o[2] = T(0); o[3] = (T)k.e_inf; o[4] = (T)k.e_inf;
+        } else {
+            size_t bn = (size_t)best;
+            bool bg = ct[bn] == CT_GHOST;
+            T fv[4];
+            for (int f = 0; f < 4; f++) fv[f] = bg ? pre_val<T>(b, r, bn, f) : ((const T*)b.U[r.out][f])[bn];
+            T rho_f = fv[0];
+            T u_f = fv[1] / rho_f, v_f = fv[2] / rho_f;
+            T ke_f = T(0.5) * rho_f * (u_f * u_f + v_f * v_f);
+            T p_f = (T)k.gm1 * (fv[3] - ke_f);
+            rho_f = tmax(rho_f, (T)k.min_rho);
+            p_f = tmax(p_f, (T)k.min_p);
+            T vn = u_f * nxs + v_f * nys;
+            T vt = -v_f * nxs + u_f * nys;
+            T T_f = p_f / (rho_f * (T)k.R);
+            T_f = tmin(T_f, (T)k.max_T);
+            T_f = tmax(T_f, (T)k.min_T);
+            p_f = rho_f * (T)k.R * T_f;
+            T ug, vg;
+            if (k.visc) {
+                ug = -u_f; vg = -v_f;
+            } else {
+                T slip = T(0);
+                if (k.wall_mode == 1) slip = T(1);
+                else if (vn > T(0)) slip = tmin(T(0.8f), vn / (vn + (T)k.slip_ref));
+                T us = -vn * nxs + vt * nys;
+                T vs = -vn * nys - vt * nxs;
+                ug = (T(1) - slip) * (-u_f) + slip * us;
+                vg = (T(1) - slip) * (-v_f) + slip * vs;
+            }
+            T vel = ::sqrt(ug * ug + vg * vg);
+            if (vel > (T)k.vghost) {
+                T s = (T)k.vghost / vel;
+                ug *= s; vg *= s;
+            }
+            T Tg;
+            if (k.adiabatic) Tg = T_f;
+            else {
+                Tg = T(2) * (T)k.T_wall - T_f;
+                Tg = tmax(Tg, (T)k.min_T);
+                Tg = tmin(Tg, (T)k.max_T);
+            }
+            T rg = tmax(p_f / ((T)k.R * Tg), (T)k.min_rho);
+            T eg = p_f / (T)k.gm1;
+            T keg = T(0.5) * rg * (ug * ug + vg * vg);
+            o[0] = rg; o[1] = rg * ug; o[2] = rg * vg; o[3] = eg + keg; o[4] = eg;
+        }
+        for (int f = 0; f < 5; f++) ((T*)b.U[r.out][f])[idx] = o[f];
+        if (FINAL) {
+            float s = (float)wave_speed(k, o);
+            if (s > 0.0f) atomicMax(&b.slots[r.slot_w * 2], __float_as_uint(s));
+            if (k.visc) {
+                T pv[5];
+                for (int f = 0; f < 5; f++) pv[f] = pre_val<T>(b, r, idx, f);
+                T u_, v_, p_, t_;
+                primitives(k, pv[0], pv[1], pv[2], pv[3], pv[4], u_, v_, p_, t_);
+                float nu = (float)(sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f)));
+                if (nu > 0.0f) atomicMax(&b.slots[r.slot_w * 2 + 1], __float_as_uint(nu));
+            }
+        }
+    }
+}
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------
+// Viscous half step (fusedViscousDiffusionKernel, src/solver.cu:2251-2511)
+// from set r.in to set r.out.  Stress and heat flux live in LDS on a 1-cell
+// ring around the tile.  FINAL: + point-wise BC + CFL maxima (mu from the
+// pre-diffusion temperature as in the reference, rho from the final state).
+// ---------------------------------------------------------------------------
+template <class T, int TX, int TY, bool FINAL>
+__global__ __launch_bounds__(NT) void k_diffuse(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
+                                                const int* __restrict__ count, Roles r) {
+    constexpr int SX = TX + 4, SY = TY + 4, QX = TX + 2, QY = TY + 2, CPT = (TX * TY) / NT;
+    static_assert(CPT * NT == TX * TY, "tile must be a multiple of the workgroup size");
+    __shared__ T s_u[SY * SX], s_v[SY * SX], s_T[SY * SX];
+    __shared__ uint8_t s_ct[SY * SX];
+    __shared__ T s_txx[QY * QX], s_tyy[QY * QX], s_txy[QY * QX], s_qx[QY * QX], s_qy[QY * QX];
+    __shared__ float s_red[2][NT / 64];
+
+    const int n_items = *count;
+    for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
+        const WorkItem it = work[w];
+        if (it.index < 0) continue;
+        const CaseDesc<T>& cd = cases[it.case_id];
+        const CaseConsts<T>& k = cd.c;
+        const int nx = k.nx, ny = k.ny;
+        const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
+        const float dtf = dt_from_slot(k, cd.b.slots, r.slot_r);
+        const T hdt = (T)(dtf * 0.5f);
+        if (it.index == 0 && threadIdx.x == 0) {
+            if (r.zero_slot) { cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u; }
+            if (r.count_step) { cd.b.t[0] += (double)dtf; cd.b.steps[0] += 1; }
+        }
+        const T* in[5];
+        T* outp[5];
+        for (int f = 0; f < 5; f++) { in[f] = (const T*)cd.b.U[r.in][f]; outp[f] = (T*)cd.b.U[r.out][f]; }
+
+        for (int n = threadIdx.x; n < SX * SY; n += NT) {
+            int li = n % SX, lj = n / SX;
+            int gi = min(max(i0 + li - 2, 0), nx - 1);
+            int gj = min(max(j0 + lj - 2, 0), ny - 1);
+            size_t g = (size_t)gj * nx + gi;
+            T u, v, p, tt;
+            primitives(k, in[0][g], in[1][g], in[2][g], in[3][g], in[4][g], u, v, p, tt);
+            s_u[n] = u; s_v[n] = v; s_T[n] = tt; s_ct[n] = cd.b.ct[g];
+        }
+        __syncthreads();
+
+        const T i2x = (T)k.inv_2dx, i2y = (T)k.inv_2dy, ix = (T)k.inv_dx, iy = (T)k.inv_dy;
+        for (int n = threadIdx.x; n < QX * QY; n += NT) {
+            int ti = n % QX, tj = n / QX;
+            int gi = i0 + ti - 1, gj = j0 + tj - 1;
+            int s = (tj + 1) * SX + (ti + 1);
+            if (gi < 0 || gi >= nx || gj < 0 || gj >= ny || s_ct[s] == CT_SOLID) {
+                s_txx[n] = T(0); s_tyy[n] = T(0); s_txy[n] = T(0); s_qx[n] = T(0); s_qy[n] = T(0);
+                continue;
+            }
+            T Tv = s_T[s];
+            T mu = sutherland(k, Tv);
+            T kk = mu * (T)k.cp * Num<T>::rcp((T)k.prandtl);
+            T dudx = (s_u[s + 1] - s_u[s - 1]) * i2x, dvdx = (s_v[s + 1] - s_v[s - 1]) * i2x;
+            T dudy = (s_u[s + SX] - s_u[s - SX]) * i2y, dvdy = (s_v[s + SX] - s_v[s - SX]) * i2y;
+            T dTdx = (s_T[s + 1] - s_T[s - 1]) * i2x, dTdy = (s_T[s + SX] - s_T[s - SX]) * i2y;
+            if (gi == 0) {
+                dudx = (s_u[s + 1] - s_u[s]) * ix; dvdx = (s_v[s + 1] - s_v[s]) * ix; dTdx = (s_T[s + 1] - s_T[s]) * ix;
+            } else if (gi == nx - 1) {
+                dudx = (s_u[s] - s_u[s - 1]) * ix; dvdx = (s_v[s] - s_v[s - 1]) * ix; dTdx = (s_T[s] - s_T[s - 1]) * ix;
+            }
+            if (gj == 0) {
+                dudy = (s_u[s + SX] - s_u[s]) * iy; dvdy = (s_v[s + SX] - s_v[s]) * iy; dTdy = (s_T[s + SX] - s_T[s]) * iy;
+            } else if (gj == ny - 1) {
+                dudy = (s_u[s] - s_u[s - SX]) * iy; dvdy = (s_v[s] - s_v[s - SX]) * iy; dTdy = (s_T[s] - s_T[s - SX]) * iy;
+            }
+            T divv = dudx + dvdy;
+            const T two3 = T(2) / T(3);
+            s_txx[n] = mu * (T(2) * dudx - two3 * divv);
+            s_tyy[n] = mu * (T(2) * dvdy - two3 * divv);
+            s_txy[n] = mu * (dudy + dvdx);
+            s_qx[n] = -kk * dTdx;
+            s_qy[n] = -kk * dTdy;
+        }
+        __syncthreads();
+
+        float smax = 0.0f, numax = 0.0f;
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            int n = threadIdx.x + q * NT;
+            int li = n % TX, lj = n / TX;
+            int gi = i0 + li, gj = j0 + lj;
+            if (gi >= nx || gj >= ny) continue;
+            size_t g = (size_t)gj * nx + gi;
+            int s = (lj + 2) * SX + (li + 2);
+            int t = (lj + 1) * QX + (li + 1);
+            uint8_t c = s_ct[s];
+            T v[5];
+            for (int f = 0; f < 5; f++) v[f] = in[f][g];
+            T Tin = s_T[s];
+            bool interior = !(c == CT_SOLID || c == CT_GHOST) && gi > 0 && gi < nx - 1 && gj > 0 && gj < ny - 1;
+            if (interior) {
+                T dtxx = (s_txx[t + 1] - s_txx[t - 1]) * i2x;
+                T dtxy_y = (s_txy[t + QX] - s_txy[t - QX]) * i2y;
+                T dtxy_x = (s_txy[t + 1] - s_txy[t - 1]) * i2x;
+                T dtyy = (s_tyy[t + QX] - s_tyy[t - QX]) * i2y;
+                T dqx = (s_qx[t + 1] - s_qx[t - 1]) * i2x;
+                T dqy = (s_qy[t + QX] - s_qy[t - QX]) * i2y;
+                T dudx = (s_u[s + 1] - s_u[s - 1]) * i2x, dudy = (s_u[s + SX] - s_u[s - SX]) * i2y;
+                T dvdx = (s_v[s + 1] - s_v[s - 1]) * i2x, dvdy = (s_v[s + SX] - s_v[s - SX]) * i2y;
+                T Phi = s_txx[t] * dudx + s_tyy[t] * dvdy + s_txy[t] * (dudy + dvdx);
+                T wxp = s_u[s + 1] * s_txx[t + 1] + s_v[s + 1] * s_txy[t + 1];
+                T wxm = s_u[s - 1] * s_txx[t - 1] + s_v[s - 1] * s_txy[t - 1];
+                T wyp = s_u[s + SX] * s_txy[t + QX] + s_v[s + SX] * s_tyy[t + QX];
+                T wym = s_u[s - SX] * s_txy[t - QX] + s_v[s - SX] * s_tyy[t - QX];
+                T rhsE = -dqx - dqy + (wxp - wxm) * i2x + (wyp - wym) * i2y;
+                T rhse = -dqx - dqy + Phi;
+                v[1] += hdt * (dtxx + dtxy_y);
+                v[2] += hdt * (dtxy_x + dtyy);
+                v[3] += hdt * rhsE;
+                v[4] += hdt * rhse;
+            }
+            if (FINAL) {
+                if (c == CT_GHOST) continue;  // filled by k_ghost
+                bc_pointwise(k, gi, gj, c, v);
+                smax = fmaxf(smax, (float)wave_speed(k, v));
+                T mu = (c == CT_SOLID) ? T(0) : sutherland(k, Tin);
+                numax = fmaxf(numax, (float)(mu * Num<T>::rcp(v[0] + T(1e-10f))));
+            }
+            for (int f = 0; f < 5; f++) outp[f][g] = v[f];
+        }
+        if (FINAL) {
+            block_max_atomic<NT>(smax, s_red[0], &cd.b.slots[r.slot_w * 2]);
+            block_max_atomic<NT>(numax, s_red[1], &cd.b.slots[r.slot_w * 2 + 1]);
+        }
+        __syncthreads();
+    }
+}
+
+// CFL maxima of set r.in over a tile list with mu from the same state
+// (reset: the reference writes mu from the post-BC initial state, solver.cu:3154).
+template <class T, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_maxima(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
+                                               const int* __restrict__ count, Roles r) {
+    __shared__ float s_red[2][NT / 64];
+    const int n_items = *count;
+    for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
+        const WorkItem it = work[w];
+        if (it.index < 0) continue;
+        const CaseDesc<T>& cd = cases[it.case_id];
+        const CaseConsts<T>& k = cd.c;
+        const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
+        float smax = 0.0f, numax = 0.0f;
+        for (int q = threadIdx.x; q < TX * TY; q += NT) {
+            int gi = i0 + q % TX, gj = j0 + q / TX;
+            if (gi >= k.nx || gj >= k.ny) continue;
+            size_t g = (size_t)gj * k.nx + gi;
+            T v[5];
+            for (int f = 0; f < 5; f++) v[f] = ((const T*)cd.b.U[r.in][f])[g];
+            T tt;
+            smax = fmaxf(smax, (float)wave_speed(k, v, &tt));
+            T mu = (cd.b.ct[g] == CT_SOLID) ? T(0) : sutherland(k, tt);
+            numax = fmaxf(numax, (float)(mu * Num<T>::rcp(v[0] + T(1e-10f))));
+        }
+        block_max_atomic<NT>(smax, s_red[0], &cd.b.slots[r.slot_w * 2]);
+        block_max_atomic<NT>(numax, s_red[1], &cd.b.slots[r.slot_w * 2 + 1]);
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(unsigned* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Derived fields of set `set` (computeTemperature/Pressure/VelocityMag/Mach-
+// DirectKernel, cellTypeToFloatKernel, density copy: solver.cu:3444-3671).
+template <class T>
+__global__ __launch_bounds__(NT) void k_field(const CaseDesc<T>* __restrict__ cases, int case_id, int set, int field, float* out) {
+    const CaseDesc<T>& cd = cases[case_id];
+    const CaseConsts<T>& k = cd.c;
+    size_t n = (size_t)k.nx * k.ny;
+    const T* rho = (const T*)cd.b.U[set][0];
+    const T* ru = (const T*)cd.b.U[set][1];
+    const T* rv = (const T*)cd.b.U[set][2];
+    const T* re = (const T*)cd.b.U[set][4];
+    for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
+        float o = 0.0f;
+        switch (field) {
+        case 0: { T e = re[i] / (rho[i] + T(1e-10f)); o = (float)((T)k.gm1 * e / (T)k.R); break; }
+        case 1: o = (float)((T)k.gm1 * re[i]); break;
+        case 2: o = (float)rho[i]; break;
+        case 3: { T r_ = rho[i] + T(1e-10f); T u = ru[i] / r_, v = rv[i] / r_; o = (float)::sqrt(u * u + v * v); break; }
+        case 4: {
+            T r_ = rho[i] + T(1e-10f); T u = ru[i] / r_, v = rv[i] / r_;
+            T sp = ::sqrt(u * u + v * v);
+            T Tt = (T)k.gm1 * (re[i] / r_) / (T)k.R;
+            T c = ::sqrt((T)k.gamma * (T)k.R * Tt);
+            o = (float)(sp / (c + T(1e-10f)));
+            break;
+        }
+        default: o = (float)cd.b.ct[i]; break;
+        }
+        out[i] = o;
+    }
+}
+
+// Statistics: max T of the primitives (mode 0) or max Mach |v|/(c+1e-10) (mode 1)
+// (launchComputeMaxTemperature / launchComputeMaxMach, solver.cu:2782-2867).
+template <class T>
+__global__ __launch_bounds__(NT) void k_stat(const CaseDesc<T>* __restrict__ cases, int case_id, int set, int mode, unsigned* target) {
+    __shared__ float s_red[NT / 64];
+    const CaseDesc<T>& cd = cases[case_id];
+    const CaseConsts<T>& k = cd.c;
+    size_t n = (size_t)k.nx * k.ny;
+    float m = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
+        T v[5];
+        for (int f = 0; f < 5; f++) v[f] = ((const T*)cd.b.U[set][f])[i];
+        T u, vv, p, tt;
+        primitives(k, v[0], v[1], v[2], v[3], v[4], u, vv, p, tt);
+        if (mode == 0) m = fmaxf(m, (float)tt);
+        else {
+            T rr = v[0] + T(1e-10f);
+            T c = ::sqrt((T)k.gamma * p / rr);
+            m = fmaxf(m, (float)(::sqrt(u * u + vv * vv) / (c + T(1e-10f))));
+        }
+    }
+    block_max_atomic<NT>(m, s_red, target);
+}
+
+// Domain totals of rho, rho_u, rho_v, E in FP64: per-workgroup partials, then
+// one workgroup sums them in a fixed order (deterministic).
+template <class T>
+__global__ __launch_bounds__(NT) void k_cons_partial(const CaseDesc<T>* __restrict__ cases, int case_id, int set, double* partial) {
+    __shared__ double s[4][NT];
+    const CaseDesc<T>& cd = cases[case_id];
+    size_t n = (size_t)cd.c.nx * cd.c.ny;
+    double acc[4] = {0, 0, 0, 0};
+    for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT)
+        for (int f = 0; f < 4; f++) acc[f] += (double)((const T*)cd.b.U[set][f])[i];
+    for (int f = 0; f < 4; f++) s[f][threadIdx.x] = acc[f];
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int f = 0; f < 4; f++) s[f][threadIdx.x] += s[f][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int f = 0; f < 4; f++) partial[blockIdx.x * 4 + f] = s[f][0];
+}
+
+__global__ __launch_bounds__(64) void k_cons_final(const double* partial, int nblocks, double* out) {
+    if (threadIdx.x < 4) {
+        double a = 0;
+        for (int b = 0; b < nblocks; b++) a += partial[b * 4 + threadIdx.x];
+        out[threadIdx.x] = a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+template <class T, int TX, int TY>
+struct Launch {
+    static void stage1(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv) {
+        k_stage<T, TX, TY, 0, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+    }
+    static void stage2(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv, bool kahan) {
+        if (kahan) k_stage<T, TX, TY, 1, true><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+        else k_stage<T, TX, TY, 1, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+    }
+    static void diffuse(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles r, bool fin) {
+        if (fin) k_diffuse<T, TX, TY, true><<<g, NT, 0, s>>>(c, w, n, r);
+        else k_diffuse<T, TX, TY, false><<<g, NT, 0, s>>>(c, w, n, r);
+    }
+    static void bc_inplace(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles r) {
+        k_bc_inplace<T, TX, TY><<<g, NT, 0, s>>>(c, w, n, r);
+    }
+    static void maxima(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles r) {
+        k_maxima<T, TX, TY><<<g, NT, 0, s>>>(c, w, n, r);
+    }
+};
+
+#define FVM_TILE_TABLE(X) X(64, 4) X(64, 8) X(32, 8) X(64, 16) X(128, 4) X(128, 8) X(32, 16)
+
+template <class T>
+static bool dispatch_tile(int tx, int ty, const KernelCall& kc) {
+#define FVM_TRY(A, B)                                                                              \
+    if (tx == A && ty == B) {                                                                      \
+        switch (kc.op) {                                                                           \
+        case OP_STAGE1: Launch<T, A, B>::stage1(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv); break; \
+        case OP_STAGE2: Launch<T, A, B>::stage2(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan); break; \
+        case OP_DIFFUSE: Launch<T, A, B>::diffuse(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.rv, false); break; \
+        case OP_DIFFUSE_FINAL: Launch<T, A, B>::diffuse(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.rv, true); break; \
+        case OP_BC_INPLACE: Launch<T, A, B>::bc_inplace(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri); break; \
+        case OP_MAXIMA: Launch<T, A, B>::maxima(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri); break; \
+        default: return false;                                                                     \
+        }                                                                                          \
+        return true;                                                                               \
+    }
+    FVM_TILE_TABLE(FVM_TRY)
+#undef FVM_TRY
+    return false;
+}
+
+bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc) {
+    return fp64 ? dispatch_tile<double>(tx, ty, kc) : dispatch_tile<float>(tx, ty, kc);
+}
+
+bool tile_supported(int tx, int ty) {
+#define FVM_HAS(A, B) if (tx == A && ty == B) return true;
+    FVM_TILE_TABLE(FVM_HAS)
+#undef FVM_HAS
+    return false;
+}
+
+void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
+                  const int* n, Roles ri, Roles rv) {
+    if (fp64) {
+        if (final_) k_ghost<double, true><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv);
+        else k_ghost<double, false><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv);
+    } else {
+        if (final_) k_ghost<float, true><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv);
+        else k_ghost<float, false><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv);
+    }
+}
+
+void launch_mark(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int fixup, int fix_set) {
+    dim3 g((unsigned)std::min<size_t>((ncells + NT - 1) / NT, 65535));
+    if (fp64) k_mark<double><<<g, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, fixup, fix_set);
+    else k_mark<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, fixup, fix_set);
+}
+
+void launch_init(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set_a, int set_b) {
+    dim3 g((unsigned)std::min<size_t>((ncells + NT - 1) / NT, 65535));
+    if (fp64) k_init<double><<<g, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set_a, set_b);
+    else k_init<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set_a, set_b);
+}
+
+void launch_copy_set(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int src, int dst) {
+    dim3 g((unsigned)std::min<size_t>((ncells + NT - 1) / NT, 65535));
+    if (fp64) k_copy_set<double><<<g, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, src, dst);
+    else k_copy_set<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, src, dst);
+}
+
+void launch_zero_u32(hipStream_t s, unsigned* p, int n) {
+    k_zero_u32<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+}
+
+void launch_field(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int field, float* out) {
+    dim3 g((unsigned)std::min<size_t>((ncells + NT - 1) / NT, 65535));
+    if (fp64) k_field<double><<<g, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set, field, out);
+    else k_field<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, field, out);
+}
+
+void launch_stat(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int mode, unsigned* target) {
+    dim3 g((unsigned)std::min<size_t>((ncells + NT - 1) / NT, 2048));
+    if (fp64) k_stat<double><<<g, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set, mode, target);
+    else k_stat<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, mode, target);
+}
+
+void launch_cons(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out) {
+    if (fp64) k_cons_partial<double><<<nblocks, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set, partial);
+    else k_cons_partial<float><<<nblocks, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, partial);
+    k_cons_final<<<1, 64, 0, s>>>(partial, nblocks, out);
+}
+
+}  // namespace fvm

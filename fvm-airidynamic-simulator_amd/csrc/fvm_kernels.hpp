@@ -1,0 +1,34 @@
+// fvm_kernels.hpp -- host-side launch interface of the kernels in fvm_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "fvm_device.hpp"
+
+namespace fvm {
+
+enum KernelOp { OP_STAGE1, OP_STAGE2, OP_DIFFUSE, OP_DIFFUSE_FINAL, OP_BC_INPLACE, OP_MAXIMA };
+
+struct KernelCall {
+    KernelOp op;
+    dim3 grid;
+    hipStream_t stream;
+    const void* cases;
+    const WorkItem* work;
+    const int* count;
+    Roles ri, rv;
+    bool kahan;
+};
+
+bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc);
+bool tile_supported(int tx, int ty);
+void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
+                  const int* n, Roles ri, Roles rv);
+void launch_mark(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int fixup, int fix_set);
+void launch_init(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set_a, int set_b);
+void launch_copy_set(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int src, int dst);
+void launch_zero_u32(hipStream_t s, unsigned* p, int n);
+void launch_field(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int field, float* out);
+void launch_stat(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int mode, unsigned* target);
+void launch_cons(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out);
+
+}  // namespace fvm
