@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark of the FVM time step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): viscous
+Navier-Stokes on a 2048x1024 grid, FP32 state, Mach 0.3, Re 200 on a 64-cell
+cylinder at (512,512), Sutherland viscosity, no-slip adiabatic wall, HLLC +
+van Leer MUSCL, SSP-RK2 with Strang viscous half steps, CFL 0.4 / 0.4 with the
+CFL dt recomputed on the device every step, +-1e-3 seeded transverse-velocity
+noise.  A step is one full time step of every cell; the step sequence is
+replayed from a captured HIP graph with the state resident in HBM.
+
+Prints ONE JSON line.  `value` = cell-updates/s over exactly --steps timed
+steps (after --warmup untimed ones), bracketed by device synchronisation.
+`roofline` = the dominant kernel's algorithmic HBM bytes per launch over its
+average launch duration, measured with HIP events on the solver's stream in a
+timing pass right after the timed region.  `cpu_baseline` = the CPU oracle
+(FP32, OpenMP) on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "fvm-airidynamic-simulator_amd"))
+
+HBM_PEAK = 8.0e12  # bytes/s, MI355X
+# Algorithmic HBM bytes per cell for each kernel of one step (FP32 state of
+# 5 fields = 20 B, cell tag 1 B): what the kernel must read and write once.
+KERNEL_BYTES_PER_CELL = {
+    "diffuse_half1": 20 + 1 + 20,        # read U^n, tag; write U_d
+    "stage1": 20 + 1 + 20,               # read U_d (or U^n), tag; write U*
+    "stage2": 20 + 20 + 1 + 20,          # read U*, U_d (blend), tag; write blend
+    "diffuse_half2": 20 + 1 + 20,        # read blend, tag; write U^{n+1}
+}
+B_ALG_PER_CELL_UPDATE = 102  # 5 x 20 B of carried state + 2 x 1 B tag per step
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=20)
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the dominant kernel from the committed rocprof
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    path = os.path.join(REPO, "profiles", "stage_kernels_2048x1024_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch", {})
+    except Exception:
+        return None
+
+
+def cpu_baseline(p, nsteps):
+    """The FP32 oracle with OpenMP on this host, nsteps of the same workload."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    threads = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
+    q = p.copy()
+    q.dt_mode = 1
+    o = O.Oracle(q, fp64=False)
+    t0 = time.perf_counter()
+    o.step(nsteps, q)
+    dt = time.perf_counter() - t0
+    o.close()
+    rate = p.nx * p.ny * nsteps / dt
+    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} steps of the {p.nx}x{p.ny} viscous workload, FP32 oracle, OpenMP, {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import fvm_amd
+    from fvm_amd import params as P
+
+    p = P.viscous_cylinder(nx=a.nx, ny=a.ny, diameter=64.0, reynolds=200.0, mach=0.3,
+                           center=(512.0, 512.0) if (a.nx, a.ny) == (2048, 1024) else None)
+    p.dt_mode = P.DT_DEVICE_CFL
+    s = fvm_amd.CFDSolver(0)
+    s.initialize(p)
+    q = p.copy()
+    s.advance(q, a.warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.advance(q, a.steps)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    cells = p.nx * p.ny
+    value = cells * a.steps / elapsed
+    ms = elapsed / a.steps * 1e3
+
+    # timing pass: per-kernel durations with HIP events on the solver's stream
+    s.enable_timing(True)
+    s.advance(q, a.timing_steps)
+    s.synchronize()
+    kt = s.kernel_timings()
+    s.enable_timing(False)
+    dominant = max(KERNEL_BYTES_PER_CELL, key=lambda k: kt.get(k, (0, 0))[0] * kt.get(k, (0, 0))[1])
+    avg_ms = kt[dominant][0]
+    alg_bytes = KERNEL_BYTES_PER_CELL[dominant] * cells
+    achieved = alg_bytes / (avg_ms * 1e-3)
+    tr = traffic_from_profiles()
+    traffic = tr.get(dominant) if isinstance(tr, dict) else None
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dominant,
+                "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": alg_bytes}
+
+    t_sim, nsteps_dev = s.time()
+    out = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"viscous Navier-Stokes cylinder {p.nx}x{p.ny}, Re 200, Mach 0.3, HLLC + van Leer, "
+                               "SSP-RK2 + Strang viscous half steps, CFL 0.4 device dt",
+                   "nx": p.nx, "ny": p.ny, "parallelism": "single GPU"},
+        "hbm_peak_fraction_at_102B": value * B_ALG_PER_CELL_UPDATE / HBM_PEAK,
+        "roofline": roofline,
+        "kernel_ms": {k: v[0] for k, v in kt.items()},
+        "sim_time": t_sim,
+    }
+    s.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, a.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -19,3 +19,7 @@ for f in fvm_kernels fvm_engine fvm_api; do grep -v "packed-fp32-ops" "$HERE/../
 hipcc --offload-arch=gfx950 -shared -fPIC -o "$OUT/libfvm_amd.so.tmp" "$HERE/../build/fvm_kernels.o" "$HERE/../build/fvm_engine.o" "$HERE/../build/fvm_api.o"
 mv "$OUT/libfvm_amd.so.tmp" "$OUT/libfvm_amd.so"
 echo "built $OUT/libfvm_amd.so"
+# KernelBenchmark counterpart (host C++ over the C ABI)
+g++ -std=c++17 -O2 -I "$HERE/../../include" "$HERE/../apps/kernel_benchmark.cpp" -o "$OUT/kernel_benchmark" \
+    -L "$OUT" -lfvm_amd -Wl,-rpath,'$ORIGIN' -Wl,-rpath,/opt/rocm/lib
+echo "built $OUT/kernel_benchmark"

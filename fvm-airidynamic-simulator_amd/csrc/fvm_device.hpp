@@ -38,8 +38,23 @@ struct CaseConsts {
     T min_rho, min_p, max_T, min_T, low_rho, vmax, vghost, slip_ref;
     T mu_ref, t_ref, s_su;
     T rho_inf, u_inf, v_inf, p_inf, e_inf, E_inf, T_wall;
+    T p_gauge;                // subtracted from every normal-momentum face flux (FP32)
     T Rinf_x[4], Rinf_y[4];   // far-field Riemann invariants (x and y normal)
     Geom geo;
+};
+
+// Static part of a ghost cell's fill, computed once per geometry: the unit
+// SDF normal and its candidate neighbours (codes 0..7 of the 8-neighbourhood
+// in the reference's scan order, solver.cu:2072) sorted by descending
+// alignment with the normal, ties kept in scan order; only neighbours inside
+// the domain with sdf > 0 and alignment > -1 are listed (3 bits each, count in
+// bits 24..27).  At run time the first listed neighbour whose density passes
+// the check is the one the reference's scan selects.
+template <class T>
+struct GhostInfo {
+    int idx;
+    uint32_t order;
+    T nx, ny;
 };
 
 struct CaseBufs {
@@ -47,6 +62,7 @@ struct CaseBufs {
     float* K[5];          // compensation carries (FP32 + Kahan variant)
     uint8_t* ct;
     float* sdf;
+    void* ghosts;         // GhostInfo<T>[n_ghosts]
     unsigned* slots;      // [2 slots][2]: bits of max wave speed, max mu/rho
     double* t;            // simulated time
     long long* steps;     // steps taken
@@ -97,6 +113,18 @@ template <> struct Num<double> {
     static __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
 };
 
+// Accurate forms (IEEE division and square root, ocml exp/log) for the
+// O(perimeter) boundary work, where the characteristic reconstruction
+// v = (R3 + R4)/2 amplifies sound-speed errors.
+template <class T> struct Precise {
+    static __device__ __forceinline__ T rcp(T x) { return T(1) / x; }
+    static __device__ __forceinline__ T div(T a, T b) { return a / b; }
+    static __device__ __forceinline__ T sqrt(T x) { return ::sqrt(x); }
+    static __device__ __forceinline__ T exp(T x) { return ::exp(x); }
+    static __device__ __forceinline__ T log(T x) { return ::log(x); }
+    static __device__ __forceinline__ bool finite(T x) { return __builtin_isfinite(x); }
+};
+
 template <class T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : (b == b ? b : a); }
 template <> __device__ __forceinline__ float tmax<float>(float a, float b) { return fmaxf(a, b); }
 template <> __device__ __forceinline__ double tmax<double>(double a, double b) { return fmax(a, b); }
@@ -127,15 +155,14 @@ __device__ __forceinline__ void primitives(const CaseConsts<T>& k, T rho, T ru, 
 }
 
 // Pressure only (fluxes need rho, u, v, p; T is not used there).
-template <class T>
+template <class T, class M = Num<T>>
 __device__ __forceinline__ void primitives_uvp(const CaseConsts<T>& k, T rho, T ru, T rv, T E, T re,
                                                T& u, T& v, T& p) {
     T r = tmax(rho, (T)k.min_rho);
-    T ir = Num<T>::rcp(r);
-    u = ru * ir;
-    v = rv * ir;
+    u = M::div(ru, r);
+    v = M::div(rv, r);
     T ke = T(0.5) * r * (u * u + v * v);
-    T eta = (E > T(0)) ? Num<T>::div(re, E) : T(1);
+    T eta = (E > T(0)) ? M::div(re, E) : T(1);
     T e = (eta < T(0.001f)) ? re : E - ke;
     p = tmax((T)k.gm1 * e, (T)k.min_p);
 }
@@ -152,11 +179,12 @@ __device__ __forceinline__ T limited_slope(int limiter, T dL, T dR) {
 }
 
 // Physical flux in face-normal form (computeFluxX, solver.cu:1005-1014).
+// pg: gauge pressure removed from the momentum flux before rounding.
 template <class T>
-__device__ __forceinline__ void phys_flux(T rho, T un, T ut, T p, T E, T f[4]) {
+__device__ __forceinline__ void phys_flux(T rho, T un, T ut, T p, T E, T f[4], T pg = T(0)) {
     T m = rho * un;
     f[0] = m;
-    f[1] = m * un + p;
+    f[1] = m * un + (p - pg);
     f[2] = m * ut;
     f[3] = (E + p) * un;
 }
@@ -191,7 +219,7 @@ __device__ __forceinline__ void hllc(const CaseConsts<T>& k, T rL, T uL, T vL, T
     bool star = !(SL >= T(0)) && !(SR <= T(0));
     T rK = useL ? rL : rR, uK = useL ? uL : uR, vK = useL ? vL : vR;
     T pK = useL ? pL : pR, EK = useL ? EL : ER, SK = useL ? SL : SR;
-    phys_flux(rK, uK, vK, pK, EK, f);
+    phys_flux(rK, uK, vK, pK, EK, f, k.p_gauge);
     if (star) {
         T dsk = SK - uK;
         T cd = SK - S_star;
@@ -240,7 +268,8 @@ __device__ __forceinline__ void ausm_plus(const CaseConsts<T>& k, T rL, T uL, T 
         Pm = MR < T(0) ? T(1) : T(0);
     }
     T m12 = Mp + Mm;
-    T p12 = Pp * pL + Pm * pR;
+    // p12 - p_gauge, rearranged so the large common part cancels before rounding
+    T p12 = Pp * (pL - k.p_gauge) + Pm * (pR - k.p_gauge) + k.p_gauge * ((Pp + Pm) - T(1));
     T mp = T(0.5) * (m12 + tabs(m12)) * c12;
     T mm = T(0.5) * (m12 - tabs(m12)) * c12;
     T aL = mp * rL, aR = mm * rR;
@@ -256,9 +285,13 @@ __device__ __forceinline__ void ausm_plus(const CaseConsts<T>& k, T rL, T uL, T 
 template <class T>
 __device__ __forceinline__ void face_flux(const CaseConsts<T>& k, const T rho[4], const T un[4], const T ut[4],
                                           const T p[4], const uint8_t ct[4], T out[5]) {
+    // The FP32 path carries the normal-momentum flux relative to a constant
+    // gauge pressure (the freestream pressure): a constant added to every face
+    // flux cancels in the divergence, and the face pressures ~p_inf no longer
+    // swamp the small momentum changes of low-Mach flow in FP32 rounding.
     auto sg = [](uint8_t c) { return c == CT_SOLID || c == CT_GHOST; };
     if (sg(ct[1]) || sg(ct[2])) {
-        out[0] = T(0); out[1] = p[1]; out[2] = T(0); out[3] = T(0); out[4] = T(0);
+        out[0] = T(0); out[1] = p[1] - k.p_gauge; out[2] = T(0); out[3] = T(0); out[4] = T(0);
         return;
     }
     bool first = sg(ct[0]) || sg(ct[3]) || rho[1] < (T)k.low_rho || rho[2] < (T)k.low_rho;
@@ -287,21 +320,21 @@ __device__ __forceinline__ void face_flux(const CaseConsts<T>& k, const T rho[4]
 }
 
 // Riemann invariants (solver.cu:680-841) in boundary-normal form.
-template <class T>
+template <class T, class M = Num<T>>
 __device__ __forceinline__ void riemann_from_prim(const CaseConsts<T>& k, T rho, T un, T ut, T p, T R[4]) {
-    T c = Num<T>::sqrt((T)k.gamma * p * Num<T>::rcp(rho));
-    R[0] = Num<T>::exp(-(T)k.gamma * Num<T>::log(rho)) * p;
+    T c = M::sqrt(M::div((T)k.gamma * p, rho));
+    R[0] = M::exp(-(T)k.gamma * M::log(rho)) * p;
     R[1] = ut;
     R[2] = un + c * (T)k.two_over_gm1;
     R[3] = un - c * (T)k.two_over_gm1;
 }
-template <class T>
+template <class T, class M = Num<T>>
 __device__ __forceinline__ void prim_from_riemann(const CaseConsts<T>& k, const T R[4], T& rho, T& un, T& ut, T& p) {
     ut = R[1];
     un = (R[2] + R[3]) * T(0.5);
     T c = (R[2] - R[3]) * (T)k.gm1_over_4;
     T c2 = c * c;
-    rho = Num<T>::exp((T)k.inv_gm1 * Num<T>::log(c2 * (T)k.inv_gamma * Num<T>::rcp(R[0])));
+    rho = M::exp((T)k.inv_gm1 * M::log(M::div(c2 * (T)k.inv_gamma, R[0])));
     p = rho * c2 * (T)k.inv_gamma;
 }
 
@@ -318,17 +351,17 @@ __device__ __forceinline__ bool bc_pointwise(const CaseConsts<T>& k, int i, int 
     if (c == CT_OUTFLOW || i == k.nx - 1) {
         T rloc = tmax(v[0], (T)k.min_rho);
         T u, vv, p;
-        primitives_uvp(k, v[0], v[1], v[2], v[3], v[4], u, vv, p);
-        T cl = Num<T>::sqrt((T)k.gamma * p * Num<T>::rcp(rloc));
+        primitives_uvp<T, Precise<T>>(k, v[0], v[1], v[2], v[3], v[4], u, vv, p);
+        T cl = Precise<T>::sqrt((T)k.gamma * p / rloc);
         T R[4];
-        riemann_from_prim(k, v[0], u, vv, p, R);
+        riemann_from_prim<T, Precise<T>>(k, v[0], u, vv, p, R);
         bool o1 = u > T(0), o3 = u + cl > T(0), o4 = u - cl > T(0);
         R[0] = o1 ? R[0] : (T)k.Rinf_x[0];
         R[1] = o1 ? R[1] : (T)k.Rinf_x[1];
         R[2] = o3 ? R[2] : (T)k.Rinf_x[2];
         R[3] = o4 ? R[3] : (T)k.Rinf_x[3];
         T rho;
-        prim_from_riemann(k, R, rho, u, vv, p);
+        prim_from_riemann<T, Precise<T>>(k, R, rho, u, vv, p);
         v[0] = rho; v[1] = rho * u; v[2] = rho * vv;
         v[4] = p * (T)k.inv_gm1;
         v[3] = v[4] + T(0.5) * rho * (u * u + vv * vv);
@@ -340,18 +373,18 @@ __device__ __forceinline__ bool bc_pointwise(const CaseConsts<T>& k, int i, int 
         touched = true;
         T rloc = tmax(v[0], (T)k.min_rho);
         T u, vv, p;
-        primitives_uvp(k, rloc, v[1], v[2], v[3], v[4], u, vv, p);
-        T cl = Num<T>::sqrt((T)k.gamma * p * Num<T>::rcp(rloc));
+        primitives_uvp<T, Precise<T>>(k, rloc, v[1], v[2], v[3], v[4], u, vv, p);
+        T cl = Precise<T>::sqrt((T)k.gamma * p / rloc);
         bool o1, o3, o4;
         if (side == 0) { o1 = vv < T(0); o3 = vv + cl < T(0); o4 = vv - cl < T(0); }
         else { o1 = vv > T(0); o3 = vv + cl > T(0); o4 = vv - cl > T(0); }
         T R[4];
-        riemann_from_prim(k, rloc, vv, u, p, R);
+        riemann_from_prim<T, Precise<T>>(k, rloc, vv, u, p, R);
         R[0] = o1 ? R[0] : (T)k.Rinf_y[0];
         R[1] = o1 ? R[1] : (T)k.Rinf_y[1];
         R[2] = o3 ? R[2] : (T)k.Rinf_y[2];
         R[3] = o4 ? R[3] : (T)k.Rinf_y[3];
-        prim_from_riemann(k, R, rloc, vv, u, p);
+        prim_from_riemann<T, Precise<T>>(k, R, rloc, vv, u, p);
         v[0] = rloc; v[1] = rloc * u; v[2] = rloc * vv;
         v[4] = p * (T)k.inv_gm1;
         v[3] = v[4] + T(0.5) * rloc * (u * u + vv * vv);
@@ -396,6 +429,15 @@ __device__ __forceinline__ float dt_from_slot(const CaseConsts<T>& k, const unsi
 
 // Wave-wide max (64 lanes), then across the workgroup through LDS, then one
 // atomic max on the bit pattern (non-negative floats order like their bits).
+// The slot only grows during a kernel, so a workgroup whose maximum does not
+// exceed the value already there skips the atomic: thousands of workgroups
+// would otherwise serialise on one address.
+__device__ __forceinline__ void slot_max(unsigned* target, float m) {
+    if (!(m > 0.0f)) return;
+    unsigned cur = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (__float_as_uint(m) > cur) atomicMax(target, __float_as_uint(m));
+}
+
 template <int NT>
 __device__ __forceinline__ void block_max_atomic(float v, float* lds_scratch, unsigned* target) {
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
@@ -405,7 +447,7 @@ __device__ __forceinline__ void block_max_atomic(float v, float* lds_scratch, un
     if (threadIdx.x == 0) {
         float m = lds_scratch[0];
         for (int w = 1; w < NT / 64; w++) m = fmaxf(m, lds_scratch[w]);
-        if (m > 0.0f) atomicMax(target, __float_as_uint(m));
+        slot_max(target, m);
     }
 }
 

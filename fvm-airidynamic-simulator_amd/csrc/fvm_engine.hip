@@ -91,8 +91,13 @@ Engine::~Engine() {
     drop_graphs();
     for (HostCase* c : cases_) {
         if (!c) continue;
-        if (use_arena_) arena_.release(c->mem);
-        else if (c->mem) (void)hipFree(c->mem);
+        if (use_arena_) {
+            arena_.release(c->mem);
+            arena_.release(c->ghost_mem);
+        } else {
+            if (c->mem) (void)hipFree(c->mem);
+            if (c->ghost_mem) (void)hipFree(c->ghost_mem);
+        }
         delete c;
     }
     if (d_cases_) (void)hipFree(d_cases_);
@@ -155,6 +160,7 @@ HostCase* Engine::get(int id) {
 }
 
 int Engine::current_set(const HostCase& c) const {
+    if (c.frozen_set >= 0) return c.frozen_set;
     if (c.p.enable_viscosity) return 0;
     return phase_ ? 2 : 0;
 }
@@ -170,6 +176,7 @@ int Engine::retire_finished() {
     int n = 0;
     for (HostCase* c : cases_)
         if (c && c->active && !c->finished && c->steps_total > 0 && c->steps_done >= c->steps_total) {
+            c->frozen_set = current_set(*c);
             c->finished = true;
             n++;
         }
@@ -273,6 +280,8 @@ static void fill_consts(CaseConsts<T>& k, const fvm_params& p) {
     T ke_inf = T(0.5) * k.rho_inf * (k.u_inf * k.u_inf + k.v_inf * k.v_inf);
     k.E_inf = k.e_inf + ke_inf;
     k.T_wall = (T)p.T_wall;
+    // FP64 keeps the reference's flux form; FP32 fluxes are taken relative to p_inf
+    k.p_gauge = (sizeof(T) == 4) ? k.p_inf : T(0);
     // far-field Riemann invariants (computeFarfieldRiemannInvarX/Y, solver.cu:799-841)
     T c_inf = std::sqrt(k.gamma * k.p_inf / k.rho_inf);
     T R0 = std::exp(-k.gamma * std::log(k.rho_inf)) * k.p_inf;
@@ -330,6 +339,7 @@ int Engine::upload_desc(HostCase& hc) {
     if (kahan_)
         for (int f = 0; f < 5; f++) b.K[f] = (float*)(base + L.k_off + round256(L.n * 4) * f);
     b.ct = (uint8_t*)(base + L.ct_off);
+    b.ghosts = hc.ghost_mem;
     b.sdf = (float*)(base + L.sdf_off);
     b.slots = (unsigned*)(base + L.slots_off);
     b.t = (double*)(base + L.t_off);
@@ -436,8 +446,13 @@ int Engine::remove_case(int id) {
     HostCase* hc = get(id);
     if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
     FVM_HIP(hipStreamSynchronize(stream_));
-    if (use_arena_) arena_.release(hc->mem);
-    else FVM_HIP(hipFree(hc->mem));
+    if (use_arena_) {
+        arena_.release(hc->mem);
+        arena_.release(hc->ghost_mem);
+    } else {
+        FVM_HIP(hipFree(hc->mem));
+        if (hc->ghost_mem) FVM_HIP(hipFree(hc->ghost_mem));
+    }
     delete hc;
     cases_[id] = nullptr;
     lists_dirty_ = true;
@@ -484,6 +499,8 @@ int Engine::reset_case(int id, const fvm_params& p) {
     if (p.nx != hc->p.nx || p.ny != hc->p.ny) return set_error(FVM_ERR_INVALID_ARGUMENT, "reset cannot change the grid size");
     if (p.enable_viscosity && hc->nsets < 4) return set_error(FVM_ERR_INVALID_ARGUMENT, "case was admitted without the viscous buffer set");
     hc->p = p;
+    hc->finished = false;
+    hc->frozen_set = -1;
     int rc = upload_desc(*hc);
     if (rc) return rc;
     size_t n = (size_t)p.nx * p.ny;
@@ -501,7 +518,6 @@ int Engine::reset_case(int id, const fvm_params& p) {
     rc = mark_and_lists(*hc, 0);
     if (rc) return rc;
     hc->steps_done = 0;
-    hc->finished = false;
     FVM_HIP(hipStreamSynchronize(stream_));
     return FVM_OK;
 }
@@ -519,12 +535,40 @@ int Engine::mark_and_lists(HostCase& hc, int fixup) {
     hc.ghosts.clear();
     for (size_t i = 0; i < n; i++)
         if (ct[i] == CT_GHOST) hc.ghosts.push_back((int)i);
+    // device table of ghost cells: cell index now, normal and ranked neighbours from k_ghost_prep
+    const size_t gsz = fp64_ ? sizeof(GhostInfo<double>) : sizeof(GhostInfo<float>);
+    if (hc.ghosts.size() > hc.ghost_cap) {
+        if (hc.ghost_mem) {
+            FVM_HIP(hipStreamSynchronize(stream_));
+            if (use_arena_) arena_.release(hc.ghost_mem);
+            else FVM_HIP(hipFree(hc.ghost_mem));
+            hc.ghost_mem = nullptr;
+            hc.ghost_cap = 0;
+        }
+        size_t cap = hc.ghosts.size() + hc.ghosts.size() / 4 + 16;
+        if (use_arena_) {
+            hc.ghost_mem = arena_.alloc(cap * gsz);
+            if (!hc.ghost_mem) return set_error(FVM_ERR_CAPACITY, "device memory budget exhausted (ghost table)");
+        } else {
+            FVM_HIP(hipMalloc(&hc.ghost_mem, cap * gsz));
+        }
+        hc.ghost_cap = cap;
+        hc.bytes = L.total + cap * gsz;
+        int rc0 = upload_desc(hc);
+        if (rc0) return rc0;
+    }
+    if (!hc.ghosts.empty()) {
+        std::vector<char> gtab(hc.ghosts.size() * gsz, 0);
+        for (size_t g = 0; g < hc.ghosts.size(); g++) std::memcpy(gtab.data() + g * gsz, &hc.ghosts[g], sizeof(int));
+        FVM_HIP(hipMemcpyAsync(hc.ghost_mem, gtab.data(), gtab.size(), hipMemcpyHostToDevice, stream_));
+        FVM_HIP(hipStreamSynchronize(stream_));
+    }
 
     int cur = current_set(hc);
     std::vector<WorkItem> tiles, ordered, gl;
     for (int t = 0; t < hc.tiles_x * hc.tiles_y; t++) tiles.push_back(WorkItem{hc.id, t});
     xcd_order(tiles, ordered);
-    for (int g : hc.ghosts) gl.push_back(WorkItem{hc.id, g});
+    for (int g = 0; g < (int)hc.ghosts.size(); g++) gl.push_back(WorkItem{hc.id, g});
     // per-case lists for the reset-path launches
     WorkItem* d_t = nullptr;
     WorkItem* d_g = nullptr;
@@ -548,6 +592,7 @@ int Engine::mark_and_lists(HostCase& hc, int fixup) {
     kc.grid = dim3((unsigned)std::max<size_t>(1, ordered.size()));
     kc.op = OP_BC_INPLACE;
     launch_tiled(fp64_, tx_, ty_, kc);
+    launch_ghost_prep(fp64_, stream_, d_cases_, d_g, (int)gl.size());
     if (!gl.empty())
         launch_ghost(fp64_, false, dim3((unsigned)((gl.size() + 255) / 256)), stream_, d_cases_, d_g, d_cnt + 1, r, r);
     // CFL maxima of the reset state into the slot the next step reads
@@ -630,7 +675,7 @@ int Engine::rebuild_lists() {
             if (c->p.enable_viscosity) visc.push_back(WorkItem{c->id, t});
         }
         if (c->p.enable_viscosity) any_visc_ = true;
-        for (int g : c->ghosts) ghosts.push_back(WorkItem{c->id, g});
+        for (int g = 0; g < (int)c->ghosts.size(); g++) ghosts.push_back(WorkItem{c->id, g});
     }
     xcd_order(all, oall);
     xcd_order(visc, ovisc);

@@ -52,11 +52,14 @@ struct HostCase {
     int nsets = 0;
     size_t bytes = 0;
     void* mem = nullptr;
-    std::vector<int> ghosts;
+    std::vector<int> ghosts;      // cell index of every ghost cell
+    void* ghost_mem = nullptr;    // GhostInfo<T>[capacity] on the device
+    size_t ghost_cap = 0;
     int tiles_x = 0, tiles_y = 0;
     int steps_total = 0;   // batch: steps to run before completion (0 = unbounded)
     long long steps_done = 0;
     bool finished = false;
+    int frozen_set = -1;   // set holding the final state once finished (-1: follows the step parity)
 };
 
 struct KernelStat {

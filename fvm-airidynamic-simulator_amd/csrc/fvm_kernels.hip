@@ -33,7 +33,7 @@ __device__ float dist_to_segment(float px, float py, float ax, float ay, float b
     t = fmaxf(0.0f, fminf(1.0f, t));
     float cx = ax + t * abx, cy = ay + t * aby;
     float dx = px - cx, dy = py - cy;
-    return __fsqrt_rn(dx * dx + dy * dy);
+    return sqrtf(dx * dx + dy * dy);
 }
 
 __device__ __forceinline__ float cross2d(float ax, float ay, float bx, float by) { return ax * by - ay * bx; }
@@ -61,7 +61,7 @@ __device__ float shape_sdf(const Geom& k, float px, float py) {
     switch (k.shape) {
     default:
     case 0:  // circle
-        return __fsqrt_rn((px - cx) * (px - cx) + (py - cy) * (py - cy)) - r;
+        return sqrtf((px - cx) * (px - cx) + (py - cy) * (py - cy)) - r;
     case 1: {  // five-point star: distance to the 10 edges, winding-number sign
         float minDist = 1e10f, winding = 0.0f;
         for (int i = 0; i < 10; i++) {
@@ -83,7 +83,7 @@ __device__ float shape_sdf(const Geom& k, float px, float py) {
         float hw = r * 1.5f, cr = r * 0.5f;
         float clampedX = fmaxf(-hw + cr, fminf(hw - cr, qx));
         float dx = qx - clampedX, dy = qy;
-        return __fsqrt_rn(dx * dx + dy * dy) - cr;
+        return sqrtf(dx * dx + dy * dy) - cr;
     }
     case 4: {  // triangle pointing along +x
         const float s32 = 0.866025404f;
@@ -99,7 +99,7 @@ __device__ float shape_sdf(const Geom& k, float px, float py) {
         return inside ? -m : m;
     }
     case 5: {  // starship: circle body, fixed wing root, rotated wing (common.h:28-36)
-        float distToCircle = __fsqrt_rn(qx * qx + qy * qy) - r;
+        float distToCircle = sqrtf(qx * qx + qy * qy) - r;
         float scale = __fdiv_rn(r, 90.0f);
         float refX = __fdiv_rn(qx, scale);
         float refY = __fdiv_rn(fabsf(qy), scale);
@@ -454,6 +454,65 @@ __device__ __forceinline__ T pre_val(const CaseBufs& b, const Roles& r, size_t n
 }
 
 #pragma clang fp contract(off)
+__constant__ int c_ox[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+__constant__ int c_oy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+
+// Normal and ranked neighbour list of each ghost cell (static per geometry).
+template <class T>
+__global__ __launch_bounds__(NT) void k_ghost_prep(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ items,
+                                                   int n_items) {
+    const int w = blockIdx.x * NT + threadIdx.x;
+    if (w >= n_items) return;
+    const WorkItem it = items[w];
+    const CaseDesc<T>& cd = cases[it.case_id];
+    const CaseConsts<T>& k = cd.c;
+    GhostInfo<T>* gi = (GhostInfo<T>*)cd.b.ghosts + it.index;
+    const int nx = k.nx, ny = k.ny;
+    const size_t idx = (size_t)gi->idx;
+    const int i = (int)(idx % nx), j = (int)(idx / nx);
+    const float* sdf = cd.b.sdf;
+    float sxp = (i < nx - 1) ? sdf[idx + 1] : sdf[idx];
+    float sxm = (i > 0) ? sdf[idx - 1] : sdf[idx];
+    float syp = (j < ny - 1) ? sdf[idx + nx] : sdf[idx];
+    float sym = (j > 0) ? sdf[idx - nx] : sdf[idx];
+    T nxs = ((T)sxp - (T)sxm) / ((T)2 * k.dx);
+    T nys = ((T)syp - (T)sym) / ((T)2 * k.dy);
+    T nl = ::sqrt(nxs * nxs + nys * nys) + (T)1e-10f;
+    nxs = nxs / nl;
+    nys = nys / nl;
+    T dots[8];
+    int codes[8];
+    int cnt = 0;
+    for (int m = 0; m < 8; m++) {
+        int ni = i + c_ox[m], nj = j + c_oy[m];
+        if (ni < 0 || ni >= nx || nj < 0 || nj >= ny) continue;
+        if (!(sdf[(size_t)nj * nx + ni] > 0.0f)) continue;
+        T dxn = (T)c_ox[m], dyn = (T)c_oy[m];
+        T len = ::sqrt(dxn * dxn + dyn * dyn);
+        T dot = (dxn * nxs + dyn * nys) / len;
+        if (!(dot > T(-1))) continue;
+        // insertion: descending dot, earlier scan position first on ties
+        int q = cnt++;
+        while (q > 0 && dots[q - 1] < dot) {
+            dots[q] = dots[q - 1];
+            codes[q] = codes[q - 1];
+            q--;
+        }
+        dots[q] = dot;
+        codes[q] = m;
+    }
+    uint32_t order = (uint32_t)cnt << 24;
+    for (int q = 0; q < cnt; q++) order |= (uint32_t)codes[q] << (3 * q);
+    gi->order = order;
+    gi->nx = nxs;
+    gi->ny = nys;
+}
+
+// Immersed-boundary ghost fill (applyBoundaryConditionsKernel ghost branch,
+// src/solver.cu:2045-2225) over a list of ghost cells.  A neighbour that is a
+// ghost cell is read at its pre-BC value (set pre0, or the SSP blend of pre0
+// and pre1); any other neighbour at its final value in set out.
+// FINAL: also the CFL maxima of the filled cells.
 template <class T, bool FINAL>
 __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ items,
                                               const int* __restrict__ count, Roles ri, Roles rv) {
@@ -464,97 +523,84 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
         const CaseConsts<T>& k = cd.c;
         const Roles r = k.visc ? rv : ri;
         const CaseBufs& b = cd.b;
-        const int nx = k.nx, ny = k.ny;
-        const size_t idx = (size_t)it.index;
-        const int i = (int)(idx % nx), j = (int)(idx / nx);
-        const float* sdf = b.sdf;
+        const GhostInfo<T> gi = ((const GhostInfo<T>*)b.ghosts)[it.index];
+        const int nx = k.nx;
+        const size_t idx = (size_t)gi.idx;
         const uint8_t* ct = b.ct;
-        const T* out0 = (const T*)b.U[r.out][0];
-        float sxp = (i < nx - 1) ? sdf[idx + 1] : sdf[idx];
-        float sxm = (i > 0) ? sdf[idx - 1] : sdf[idx];
-        float syp = (j < ny - 1) ? sdf[idx + nx] : sdf[idx];
-        float sym = (j > 0) ? sdf[idx - nx] : sdf[idx];
-        T nxs = (T)((T)sxp - (T)sxm) / ((T)2 * (T)k.dx);
-        T nys = (T)((T)syp - (T)sym) / ((T)2 * (T)k.dy);
-        T nl = ::sqrt(nxs * nxs + nys * nys) + (T)1e-10f;
-        nxs = nxs / nl;
-        nys = nys / nl;
-        const int ox[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
-        const int oy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+        const T nxs = gi.nx, nys = gi.ny;
+        const int cnt = (int)(gi.order >> 24);
         long long best = -1;
-        T best_dot = T(-1);
-        for (int m = 0; m < 8; m++) {
-            int ni = i + ox[m], nj = j + oy[m];
-            if (ni < 0 || ni >= nx || nj < 0 || nj >= ny) continue;
-            size_t nidx = (size_t)nj * nx + ni;
-            if (!(sdf[nidx] > 0.0f)) continue;
-            T rn = (ct[nidx] == CT_GHOST) ? pre_val<T>(b, r, nidx, 0) : out0[nidx];
-            if (!(rn >= (T)k.min_rho)) continue;
-            T dxn = (T)ox[m], dyn = (T)oy[m];
-            T len = ::sqrt(dxn * dxn + dyn * dyn);
-            T dot = (dxn * nxs + dyn * nys) / len;
-            if (dot > best_dot) { best_dot = dot; best = (long long)nidx; }
+        bool best_ghost = false;
+        for (int q = 0; q < cnt; q++) {
+            int m = (gi.order >> (3 * q)) & 7;
+            size_t nidx = (size_t)((long long)idx + c_oy[m] * (long long)nx + c_ox[m]);
+            bool g = ct[nidx] == CT_GHOST;
+            T rn = g ? pre_val<T>(b, r, nidx, 0) : ((const T*)b.U[r.out][0])[nidx];
+            if (rn >= k.min_rho) {
+                best = (long long)nidx;
+                best_ghost = g;
+                break;
+            }
         }
         T o[5];
         if (best < 0) {
-            o[0] = (T)k.rho_inf; o[1] = T(0); o[2] = T(0); o[3] = (T)k.e_inf; o[4] = (T)k.e_inf;
+            o[0] = k.rho_inf; o[1] = T(0); o[2] = T(0); o[3] = k.e_inf; o[4] = k.e_inf;
         } else {
             size_t bn = (size_t)best;
-            bool bg = ct[bn] == CT_GHOST;
             T fv[4];
-            for (int f = 0; f < 4; f++) fv[f] = bg ? pre_val<T>(b, r, bn, f) : ((const T*)b.U[r.out][f])[bn];
+            for (int f = 0; f < 4; f++) fv[f] = best_ghost ? pre_val<T>(b, r, bn, f) : ((const T*)b.U[r.out][f])[bn];
             T rho_f = fv[0];
             T u_f = fv[1] / rho_f, v_f = fv[2] / rho_f;
             T ke_f = T(0.5) * rho_f * (u_f * u_f + v_f * v_f);
-            T p_f = (T)k.gm1 * (fv[3] - ke_f);
-            rho_f = tmax(rho_f, (T)k.min_rho);
-            p_f = tmax(p_f, (T)k.min_p);
+            T p_f = k.gm1 * (fv[3] - ke_f);
+            rho_f = tmax(rho_f, k.min_rho);
+            p_f = tmax(p_f, k.min_p);
             T vn = u_f * nxs + v_f * nys;
             T vt = -v_f * nxs + u_f * nys;
-            T T_f = p_f / (rho_f * (T)k.R);
-            T_f = tmin(T_f, (T)k.max_T);
-            T_f = tmax(T_f, (T)k.min_T);
-            p_f = rho_f * (T)k.R * T_f;
+            T T_f = p_f / (rho_f * k.R);
+            T_f = tmin(T_f, k.max_T);
+            T_f = tmax(T_f, k.min_T);
+            p_f = rho_f * k.R * T_f;
             T ug, vg;
             if (k.visc) {
                 ug = -u_f; vg = -v_f;
             } else {
                 T slip = T(0);
                 if (k.wall_mode == 1) slip = T(1);
-                else if (vn > T(0)) slip = tmin(T(0.8f), vn / (vn + (T)k.slip_ref));
+                else if (vn > T(0)) slip = tmin(T(0.8f), vn / (vn + k.slip_ref));
                 T us = -vn * nxs + vt * nys;
                 T vs = -vn * nys - vt * nxs;
                 ug = (T(1) - slip) * (-u_f) + slip * us;
                 vg = (T(1) - slip) * (-v_f) + slip * vs;
             }
             T vel = ::sqrt(ug * ug + vg * vg);
-            if (vel > (T)k.vghost) {
-                T s = (T)k.vghost / vel;
+            if (vel > k.vghost) {
+                T s = k.vghost / vel;
                 ug *= s; vg *= s;
             }
             T Tg;
             if (k.adiabatic) Tg = T_f;
             else {
-                Tg = T(2) * (T)k.T_wall - T_f;
-                Tg = tmax(Tg, (T)k.min_T);
-                Tg = tmin(Tg, (T)k.max_T);
+                Tg = T(2) * k.T_wall - T_f;
+                Tg = tmax(Tg, k.min_T);
+                Tg = tmin(Tg, k.max_T);
             }
-            T rg = tmax(p_f / ((T)k.R * Tg), (T)k.min_rho);
-            T eg = p_f / (T)k.gm1;
+            T rg = tmax(p_f / (k.R * Tg), k.min_rho);
+            T eg = p_f / k.gm1;
             T keg = T(0.5) * rg * (ug * ug + vg * vg);
             o[0] = rg; o[1] = rg * ug; o[2] = rg * vg; o[3] = eg + keg; o[4] = eg;
         }
         for (int f = 0; f < 5; f++) ((T*)b.U[r.out][f])[idx] = o[f];
         if (FINAL) {
             float s = (float)wave_speed(k, o);
-            if (s > 0.0f) atomicMax(&b.slots[r.slot_w * 2], __float_as_uint(s));
+            slot_max(&b.slots[r.slot_w * 2], s);
             if (k.visc) {
                 T pv[5];
                 for (int f = 0; f < 5; f++) pv[f] = pre_val<T>(b, r, idx, f);
                 T u_, v_, p_, t_;
                 primitives(k, pv[0], pv[1], pv[2], pv[3], pv[4], u_, v_, p_, t_);
                 float nu = (float)(sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f)));
-                if (nu > 0.0f) atomicMax(&b.slots[r.slot_w * 2 + 1], __float_as_uint(nu));
+                slot_max(&b.slots[r.slot_w * 2 + 1], nu);
             }
         }
     }
@@ -881,6 +927,13 @@ void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* 
         if (final_) k_ghost<float, true><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv);
         else k_ghost<float, false><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv);
     }
+}
+
+void launch_ghost_prep(bool fp64, hipStream_t s, const void* c, const WorkItem* items, int n) {
+    if (n <= 0) return;
+    dim3 g((unsigned)((n + NT - 1) / NT));
+    if (fp64) k_ghost_prep<double><<<g, NT, 0, s>>>((const CaseDesc<double>*)c, items, n);
+    else k_ghost_prep<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, items, n);
 }
 
 void launch_mark(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int fixup, int fix_set) {

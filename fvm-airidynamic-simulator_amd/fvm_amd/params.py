@@ -190,3 +190,48 @@ def viscous_cylinder(nx: int = 2048, ny: int = 1024, diameter: float = 64.0, rey
     p.obstacle_y = cy
     p.dt = 0.3
     return p
+
+
+def mixed_case(rng, n_lo=64 * 64, n_hi=512 * 256, nx_range=(64, 512), ny_range=(64, 256),
+               shapes=(SHAPE_CIRCLE, SHAPE_STAR, SHAPE_DIAMOND, SHAPE_CAPSULE, SHAPE_TRIANGLE)) -> FvmParams:
+    """One case of the mixed batch: cells log-uniform in [n_lo, n_hi], aspect U[1,2],
+    shape uniform over circle/star/diamond/capsule/triangle, size 8-25% of the height,
+    random rotation, centre (W/4, H/2), Mach U[0.3, 4], viscous (Re U[100,1000]) or
+    inviscid 50/50, HLLC + minmod, CFL 0.5 / 0.4, device CFL dt, FP32."""
+    n = math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))
+    a = rng.uniform(1.0, 2.0)
+    ny = int(round(math.sqrt(n / a)))
+    nx = int(round(a * ny))
+    nx = min(max(nx, nx_range[0]), nx_range[1])
+    ny = min(max(ny, ny_range[0]), ny_range[1])
+    shape = int(shapes[rng.integers(len(shapes))])
+    s = rng.uniform(0.08, 0.25)
+    rot = rng.uniform(0.0, 2 * math.pi)
+    mach = rng.uniform(0.3, 4.0)
+    visc = bool(rng.integers(2))
+    seed = int(rng.integers(1, 2 ** 31))
+    if visc:
+        re = rng.uniform(100.0, 1000.0)
+        p = viscous_cylinder(nx=nx, ny=ny, diameter=2.0 * s * ny, reynolds=re, mach=mach, seed=seed,
+                             limiter=LIMITER_MINMOD, center=(nx / 4.0, ny / 2.0))
+    else:
+        p = euler_bow_shock(nx=nx, ny=ny, seed=seed, mach=mach, radius=s * ny, center=(nx / 4.0, ny / 2.0))
+    p.obstacle_shape = shape
+    p.obstacle_rotation = rot
+    p.cfl, p.cfl_visc = 0.5, 0.4
+    p.dt_mode = DT_DEVICE_CFL
+    return p
+
+
+def mixed_batch(n_cases: int = 512, seed: int = 4) -> list:
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [mixed_case(rng) for _ in range(n_cases)]
+
+
+def served_request(rng) -> tuple:
+    """A served solve request: grid log-uniform 128x64 .. 2048x1024, other
+    parameters as the mixed batch, steps uniform in 500..5000."""
+    p = mixed_case(rng, n_lo=128 * 64, n_hi=2048 * 1024, nx_range=(128, 2048), ny_range=(64, 1024))
+    steps = int(rng.integers(500, 5001))
+    return p, steps

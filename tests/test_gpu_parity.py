@@ -1,0 +1,215 @@
+"""HIP path vs the CPU oracle, through the C ABI, on seeded inputs.
+
+Bars: cell-type masks and SDF bit-exact; FP32 fields within 1e-4 relative L1
+of the FP64 oracle after 200 steps (Linf reported); the FP64 GPU path within
+1e-9 of the FP64 oracle; graph replay, tile shape and batching bitwise
+invariant.
+"""
+import numpy as np
+import pytest
+
+import fvm_amd
+import oracle as O
+from fvm_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+L1_BAR = 1e-4
+
+
+def rel_errors(g, r):
+    out = []
+    for c in range(5):
+        l1 = np.abs(g[c] - r[c]).sum() / np.abs(r[c]).sum()
+        li = np.abs(g[c] - r[c]).max() / np.abs(r[c]).max()
+        out.append((l1, li))
+    return out
+
+
+def announce(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = fvm_amd.CFDSolver(0)
+    yield s
+    s.close()
+
+
+@pytest.mark.parametrize("shape", range(6))
+def test_masks_bit_exact(solver, shape):
+    p = P.euler_bow_shock(nx=200, ny=120, radius=22.0, center=(70.0, 60.0))
+    p.obstacle_shape = shape
+    p.obstacle_rotation = 0.37 * shape
+    p.wing_rotation = 0.3
+    announce(f"mask shape={shape} 200x120")
+    solver.initialize(p)
+    o = O.Oracle(p, fp64=False)
+    assert np.array_equal(solver.getCellTypes(), o.cell_types())
+    assert np.array_equal(solver.getSDF(), o.sdf())
+
+
+def test_reset_state_matches(solver):
+    p = P.euler_bow_shock()
+    announce("reset state 256x128")
+    solver.initialize(p)
+    g = solver.state().astype(np.float64)
+    r = O.Oracle(p, fp64=False).state()
+    for l1, li in rel_errors(g, r):
+        assert l1 < 1e-6 and li < 1e-5
+
+
+def run_pair(solver, p, nsteps, fp64_oracle=True):
+    solver.initialize(p)
+    q = p.copy()
+    solver.advance(q, nsteps)
+    g = solver.state64()
+    o = O.Oracle(p, fp64=fp64_oracle)
+    o.step(nsteps, p.copy())
+    return g, o.state(), solver, o
+
+
+@pytest.mark.parametrize("flux,limiter", [(P.FLUX_HLLC, P.LIMITER_MINMOD), (P.FLUX_AUSM_PLUS, P.LIMITER_MINMOD),
+                                          (P.FLUX_HLLC, P.LIMITER_VAN_LEER)])
+def test_euler_bow_shock_200_steps(solver, flux, limiter):
+    p = P.euler_bow_shock(flux=flux, limiter=limiter)
+    p.dt = 0.1
+    announce(f"Euler 256x128 M3 flux={flux} limiter={limiter} 200 steps dt=0.1")
+    g, r, _, _ = run_pair(solver, p, 200)
+    errs = rel_errors(g, r)
+    print("  rel L1 / Linf per field:", ["%.2e/%.2e" % e for e in errs], flush=True)
+    assert np.isfinite(g).all()
+    for l1, _ in errs:
+        assert l1 < L1_BAR
+
+
+def test_viscous_cylinder_200_steps(solver):
+    p = P.viscous_cylinder(nx=192, ny=96, diameter=16.0, reynolds=100.0, mach=0.3)
+    p.dt = 0.3
+    announce("viscous 192x96 Re100 M0.3 HLLC+vanLeer 200 steps dt=0.3")
+    g, r, _, _ = run_pair(solver, p, 200)
+    errs = rel_errors(g, r)
+    print("  rel L1 / Linf per field:", ["%.2e/%.2e" % e for e in errs], flush=True)
+    for l1, _ in errs:
+        assert l1 < L1_BAR
+
+
+@pytest.mark.parametrize("visc", [0, 1])
+def test_fp64_path_tight(solver, visc):
+    if visc:
+        p = P.viscous_cylinder(nx=128, ny=64, diameter=12.0, reynolds=100.0, mach=0.5)
+        p.dt = 0.25
+    else:
+        p = P.euler_bow_shock(nx=128, ny=64, radius=10.0, center=(32.0, 32.0))
+        p.dt = 0.1
+    p.precision = P.PRECISION_FP64
+    announce(f"FP64 GPU vs FP64 oracle visc={visc} 50 steps")
+    g, r, _, _ = run_pair(solver, p, 50)
+    errs = rel_errors(g, r)
+    print("  rel L1 / Linf:", ["%.1e/%.1e" % e for e in errs], flush=True)
+    for l1, li in errs:
+        assert l1 < 1e-10
+
+
+def test_device_cfl_dt(solver):
+    p = P.euler_bow_shock()
+    p.dt_mode = P.DT_DEVICE_CFL
+    announce("device CFL dt, 100 steps")
+    solver.initialize(p)
+    dt0 = solver.computeStableTimeStep(p)
+    assert dt0 == pytest.approx(O.Oracle(p, fp64=False).stable_dt(), rel=1e-5)
+    q = p.copy()
+    solver.advance(q, 100)
+    t, n = solver.time()
+    assert n == 100
+    oo = O.Oracle(p, fp64=True)
+    oo.step(100, p.copy())
+    assert t == pytest.approx(oo.params.t_current, rel=1e-5)
+    for l1, _ in rel_errors(solver.state64(), oo.state()):
+        assert l1 < L1_BAR
+
+
+def test_graph_replay_bitwise_equals_direct(solver):
+    p = P.euler_bow_shock(nx=160, ny=96, radius=12.0, center=(40.0, 48.0))
+    p.dt_mode = P.DT_DEVICE_CFL
+    announce("graph replay vs direct launches, 40 steps")
+    solver.initialize(p)
+    q = p.copy()
+    solver.advance(q, 40)           # graph path
+    a = solver.state()
+    solver.reset(p)
+    q = p.copy()
+    for _ in range(40):
+        solver.step(q)              # direct launches
+    b = solver.state()
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("tile", [(64, 4), (32, 8), (64, 16), (128, 4), (128, 8), (32, 16)])
+def test_tile_shape_bitwise_invariant(solver, tile):
+    p = P.viscous_cylinder(nx=150, ny=70, diameter=12.0, reynolds=100.0, mach=0.4)
+    p.dt = 0.25
+    announce(f"tile {tile} vs 64x8, viscous 150x70, 20 steps")
+    solver.set_tile(64, 8)
+    solver.initialize(p)
+    q = p.copy()
+    solver.advance(q, 20)
+    a = solver.state()
+    solver.set_tile(*tile)
+    solver.initialize(p)
+    q = p.copy()
+    solver.advance(q, 20)
+    b = solver.state()
+    solver.set_tile(0, 0)
+    assert np.array_equal(a, b)
+
+
+def test_kahan_variant_not_worse(solver):
+    p = P.euler_bow_shock()
+    p.dt = 0.1
+    announce("FP32 vs FP32+Kahan vs FP64 oracle, 400 steps")
+    r = O.Oracle(p, fp64=True)
+    r.step(400, p.copy())
+    ref = r.state()
+    solver.initialize(p)
+    q = p.copy()
+    solver.advance(q, 400)
+    e32 = rel_errors(solver.state64(), ref)
+    pk = p.copy()
+    pk.precision = P.PRECISION_FP32_KAHAN
+    solver.initialize(pk)
+    q = pk.copy()
+    solver.advance(q, 400)
+    ek = rel_errors(solver.state64(), ref)
+    print("  plain:", ["%.2e" % e[0] for e in e32], " kahan:", ["%.2e" % e[0] for e in ek], flush=True)
+    for (a, _), (b, _) in zip(e32, ek):
+        assert b < L1_BAR
+        assert b <= 1.5 * a + 1e-7
+
+
+def test_readback_fields_and_stats(solver):
+    p = P.euler_bow_shock()
+    p.dt = 0.1
+    announce("field readback and statistics")
+    solver.initialize(p)
+    q = p.copy()
+    solver.advance(q, 30)
+    s = solver.state().astype(np.float64)
+    rho, ru, rv, E, re = s
+    g = P.f32(1.4)
+    R = P.f32(p.r_gas)
+    T = solver.field(P.FIELD_TEMPERATURE)
+    assert np.allclose(T, (g - 1) * (re / (rho + 1e-10)) / R, rtol=1e-5)
+    assert np.allclose(solver.field(P.FIELD_PRESSURE), (g - 1) * re, rtol=1e-6)
+    assert np.array_equal(solver.field(P.FIELD_DENSITY), s[0].astype(np.float32))
+    vm = np.sqrt((ru / (rho + 1e-10)) ** 2 + (rv / (rho + 1e-10)) ** 2)
+    assert np.allclose(solver.field(P.FIELD_VELOCITY_MAG), vm, rtol=1e-5)
+    assert np.array_equal(solver.field(P.FIELD_CELL_TYPE), solver.getCellTypes().astype(np.float32))
+    sums = solver.conservation_sums()
+    assert np.allclose(sums, s[:4].reshape(4, -1).sum(axis=1), rtol=1e-12)
+    o = O.Oracle(p, fp64=False)
+    o.set_state(s)
+    prim = o.prims()
+    assert solver.getMaxTemperature() == pytest.approx(prim[3].max(), rel=1e-5)
+    assert solver.getSimulationMemoryUsage() > 20 * p.nx * p.ny
