@@ -47,6 +47,13 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--batch-cases", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="requests/s of the served stream")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-step-scale", type=float, default=0.2, help="scales the 500-5000 step counts")
+    ap.add_argument("--no-serve", action="store_true")
     return ap.parse_args()
 
 
@@ -79,6 +86,54 @@ def cpu_baseline(p, nsteps):
     rate = p.nx * p.ny * nsteps / dt
     return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{nsteps} steps of the {p.nx}x{p.ny} viscous workload, FP32 oracle, OpenMP, {dt:.1f} s"}
+
+
+def batched_vs_plain(ncases, nsteps):
+    """The mixed batch (grids 64x64..512x256, five shapes, Mach 0.3-4, half
+    viscous, per-case CFL dt): all cases in the batched engine vs the same
+    cases one at a time through the plain solver, nsteps each, same GPU."""
+    import fvm_amd
+    from fvm_amd import params as P
+    from fvm_amd.batch import BatchEngine
+    cases = P.mixed_batch(ncases)
+    cells = sum(p.nx * p.ny for p in cases)
+    b = BatchEngine(0, mem_budget_bytes=16 << 30, max_cases=ncases)
+    for p in cases:
+        b.add(p)
+    b.advance(2)  # capture the step graph
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.advance(nsteps)
+    b.synchronize()
+    tb = time.perf_counter() - t0
+    b.close()
+    s = fvm_amd.CFDSolver(0)
+    tp = 0.0
+    for p in cases:
+        s.initialize(p)
+        q = p.copy()
+        s.advance(q, 2)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.advance(q, nsteps)
+        s.synchronize()
+        tp += time.perf_counter() - t0
+    s.close()
+    rb, rp = cells * nsteps / tb, cells * nsteps / tp
+    return {"cases": ncases, "cells": cells, "steps": nsteps, "batched_cell_updates_per_s": rb,
+            "plain_cell_updates_per_s": rp, "speedup": rb / rp}
+
+
+def served(rate, seconds, step_scale):
+    """Poisson request stream (grids 128x64..2048x1024, steps 500-5000 x step_scale)."""
+    from fvm_amd.serve import SolveServer, poisson_stream
+    reqs = poisson_stream(rate, seconds, seed=5, step_scale=step_scale)
+    srv = SolveServer(0, mem_budget_bytes=200 << 30, max_active=1024, chunk_steps=25)
+    st = srv.run(reqs, time_limit=seconds * 3)
+    srv.close()
+    st.pop("done")
+    st.update({"rate_per_s": rate, "arrival_window_s": seconds, "step_scale": step_scale})
+    return st
 
 
 def main():
@@ -145,6 +200,10 @@ def main():
         "sim_time": t_sim,
     }
     s.close()
+    if not a.no_batch:
+        out["batched"] = batched_vs_plain(a.batch_cases, a.batch_steps)
+    if not a.no_serve:
+        out["served"] = served(a.serve_rate, a.serve_seconds, a.serve_step_scale)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, a.cpu_steps)
     else:

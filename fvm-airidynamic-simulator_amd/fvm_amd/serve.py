@@ -1,0 +1,139 @@
+"""Continuous-batching solve server over the batched engine.
+
+Requests (grid, obstacle, flow, step count) arrive over time.  The loop
+admits queued requests whenever the device memory budget and the slot limit
+allow, folds them into the running batch (one launch per kernel for all
+resident cases; each case keeps its own CFL dt), retires a case as soon as it
+has taken its step count, reads its state back and frees its memory for the
+next admission.  State never leaves HBM between steps.  A case's arithmetic
+does not depend on its neighbours in the batch, so a result is bitwise the
+same alone and in the mix.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+from . import params as P
+from .batch import BatchEngine
+from .lib import FvmError
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of the run
+    params: P.FvmParams
+    steps: int
+    admitted: float = -1.0
+    completed: float = -1.0
+    result: np.ndarray | None = None
+
+    @property
+    def cells(self) -> int:
+        return self.params.nx * self.params.ny
+
+
+class SolveServer:
+    def __init__(self, device: int = 0, mem_budget_bytes: int = 200 << 30, max_active: int = 1024,
+                 chunk_steps: int = 25, keep_results: bool = False):
+        self.engine = BatchEngine(device, mem_budget_bytes=mem_budget_bytes, max_cases=max_active)
+        self.budget = mem_budget_bytes
+        self.max_active = max_active
+        self.chunk = chunk_steps
+        self.keep_results = keep_results
+
+    def close(self):
+        self.engine.close()
+
+    def run(self, requests: list[Request], time_limit: float | None = None) -> dict:
+        """Serve `requests` (arrival times relative to the call) until all are
+        done or `time_limit` seconds pass.  Returns latency and throughput stats."""
+        future = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        queue: collections.deque[Request] = collections.deque()
+        active: dict[int, Request] = {}
+        done: list[Request] = []
+        in_use = 0
+        sizes: dict[int, int] = {}
+        t0 = time.perf_counter()
+        busy = 0.0
+        while True:
+            now = time.perf_counter() - t0
+            if time_limit is not None and now > time_limit:
+                break
+            while future and future[0].arrival <= now:
+                queue.append(future.popleft())
+            # admission: FIFO against the memory budget and the slot limit
+            while queue and len(active) < self.max_active:
+                r = queue[0]
+                need = BatchEngine.case_bytes(r.params) + r.cells * 2  # + ghost table allowance
+                if in_use + need > self.budget:
+                    break
+                try:
+                    cid = self.engine.add(r.params, steps_total=r.steps)
+                except FvmError as e:
+                    if e.code == -5:  # budget exhausted (fragmentation): retry after a retirement
+                        break
+                    raise
+                queue.popleft()
+                r.admitted = time.perf_counter() - t0
+                active[cid] = r
+                sizes[cid] = need
+                in_use += need
+            if not active:
+                if not future and not queue:
+                    break
+                if future:
+                    wait = future[0].arrival - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+                continue
+            tb = time.perf_counter()
+            self.engine.advance(self.chunk)
+            self.engine.synchronize()
+            busy += time.perf_counter() - tb
+            for cid in list(active):
+                n, _, fin = self.engine.status(cid)
+                if fin:
+                    r = active.pop(cid)
+                    state = self.engine.state(cid)
+                    if self.keep_results:
+                        r.result = state
+                    r.completed = time.perf_counter() - t0
+                    self.engine.remove(cid)
+                    in_use -= sizes.pop(cid)
+                    done.append(r)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.completed - r.arrival for r in done]) if done else np.zeros(1)
+        work = sum(r.cells * r.steps for r in done)
+        return {
+            "completed": len(done),
+            "submitted": len(requests),
+            "left_in_queue": len(queue) + len(future),
+            "left_active": len(active),
+            "wall_s": wall,
+            "busy_s": busy,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()),
+            "cell_updates_per_s": work / wall if wall > 0 else 0.0,
+            "done": done,
+        }
+
+
+def poisson_stream(rate_per_s: float, duration_s: float, seed: int = 5, request_fn=P.served_request,
+                   step_scale: float = 1.0) -> list[Request]:
+    """Seeded Poisson arrivals at `rate_per_s` over `duration_s` seconds."""
+    rng = np.random.default_rng(seed)
+    t, out, rid = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate_per_s)
+        if t > duration_s:
+            break
+        p, steps = request_fn(rng)
+        out.append(Request(rid, t, p, max(1, int(round(steps * step_scale)))))
+        rid += 1
+    return out
