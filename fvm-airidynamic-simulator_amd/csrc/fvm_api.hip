@@ -3,7 +3,9 @@
 // fvm_solver mirrors CFDSolver (src/include/solver.cuh:22-80, src/solver.cu:
 // 2874-3514): one case, owned buffers, reset / step / dt / readback.
 // fvm_batch advances many independent cases in one launch per kernel.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -49,12 +51,28 @@ bool consts_differ(const fvm_params& a, const fvm_params& b) {
     return std::memcmp(&x, &y, sizeof(x)) != 0;
 }
 
+#include "tile_table.inc"
+
+struct TileEntry {
+    int viscous, bucket, tx, ty;
+};
+const TileEntry kTileTable[] = {FVM_TILE_ENTRIES};
+
+// Tile shape for a case from the autotuned table: same viscosity, nearest
+// log4 cell-count bucket (fvm_amd/autotune.py measures and writes the table).
 int pick_tile(const fvm_params& p, int* tx, int* ty) {
-    // Defaults from the tile sweep (stage kernels): wide rows for coalescing,
-    // few rows so halo recomputation stays small.
-    (void)p;
-    *tx = 64;
-    *ty = 8;
+    double cells = (double)p.nx * (double)p.ny;
+    int bucket = (int)std::lround(std::log(std::max(cells, 1.0) / 32768.0) / std::log(4.0));
+    bucket = std::max(0, std::min(4, bucket));
+    int best = -1, best_d = 1 << 30;
+    for (int i = 0; i < (int)(sizeof(kTileTable) / sizeof(kTileTable[0])); i++) {
+        if (kTileTable[i].viscous != (p.enable_viscosity ? 1 : 0)) continue;
+        int d = std::abs(kTileTable[i].bucket - bucket);
+        if (d < best_d) { best_d = d; best = i; }
+    }
+    *tx = best >= 0 ? kTileTable[best].tx : 64;
+    *ty = best >= 0 ? kTileTable[best].ty : 8;
+    if (!fvm::tile_supported(*tx, *ty, p.precision == FVM_PRECISION_FP64)) { *tx = 64; *ty = 8; }
     return 0;
 }
 
@@ -396,7 +414,7 @@ int fvm_set_tile(fvm_solver* s, int tile_x, int tile_y) {
         }
         return FVM_OK;
     }
-    if (!fvm::tile_supported(tile_x, tile_y)) return set_error(FVM_ERR_INVALID_ARGUMENT, "unsupported tile shape");
+    if (!fvm::tile_supported(tile_x, tile_y, false)) return set_error(FVM_ERR_INVALID_ARGUMENT, "unsupported tile shape");
     s->tile_x = tile_x;
     s->tile_y = tile_y;
     if (s->eng) return s->eng->set_tile(tile_x, tile_y);
@@ -440,6 +458,8 @@ int fvm_batch_create(int device, size_t mem_budget_bytes, int max_cases, fvm_bat
     b->max_cases = max_cases;
     b->eng.reset(new Engine(device, false, false));
     int rc = b->eng->init_stream();
+    if (rc) return rc;
+    rc = b->eng->set_tile(FVM_BATCH_TILE_X, FVM_BATCH_TILE_Y);
     if (rc) return rc;
     rc = b->eng->set_arena(mem_budget_bytes);
     if (rc) return rc;
@@ -537,6 +557,15 @@ int fvm_batch_bytes_in_use(fvm_batch* b, size_t* bytes) {
 int fvm_batch_synchronize(fvm_batch* b) {
     if (!b) return set_error(FVM_ERR_INVALID_ARGUMENT, "null batch");
     return b->eng->sync();
+}
+
+int fvm_batch_set_tile(fvm_batch* b, int tile_x, int tile_y) {
+    if (!b) return set_error(FVM_ERR_INVALID_ARGUMENT, "null batch");
+    if (tile_x == 0 && tile_y == 0) {
+        tile_x = FVM_BATCH_TILE_X;
+        tile_y = FVM_BATCH_TILE_Y;
+    }
+    return b->eng->set_tile(tile_x, tile_y);
 }
 
 int fvm_batch_use_graph(fvm_batch* b, int enable) {

@@ -39,7 +39,7 @@ struct CaseConsts {
     T mu_ref, t_ref, s_su;
     T rho_inf, u_inf, v_inf, p_inf, e_inf, E_inf, T_wall;
     T p_gauge;                // subtracted from every normal-momentum face flux (FP32)
-    T Rinf_x[4], Rinf_y[4];   // far-field Riemann invariants (x and y normal)
+    T s_far, c_far;           // far-field entropy p rho^-gamma and sound speed
     Geom geo;
 };
 
@@ -99,6 +99,7 @@ template <> struct Num<float> {
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
     static __device__ __forceinline__ float exp(float x) { return __expf(x); }
     static __device__ __forceinline__ float log(float x) { return __logf(x); }
     static __device__ __forceinline__ bool finite(float x) { return __builtin_isfinite(x); }
@@ -108,6 +109,7 @@ template <> struct Num<double> {
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
     static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / ::sqrt(x); }
     static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
     static __device__ __forceinline__ double log(double x) { return ::log(x); }
     static __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
@@ -146,8 +148,8 @@ __device__ __forceinline__ void primitives(const CaseConsts<T>& k, T rho, T ru, 
     u = ru * ir;
     v = rv * ir;
     T ke = T(0.5) * r * (u * u + v * v);
-    T eta = (E > T(0)) ? Num<T>::div(re, E) : T(1);
-    T e = (eta < T(0.001f)) ? re : E - ke;
+    // dual-energy switch eta = rho_e / E < 1e-3 (E > 0), as a multiply-compare
+    T e = (E > T(0) && re < T(0.001f) * E) ? re : E - ke;
     p = tmax((T)k.gm1 * e, (T)k.min_p);
     T tt = p * ir * (T)k.inv_R;
     tt = tmin(tt, (T)k.max_T);
@@ -162,8 +164,7 @@ __device__ __forceinline__ void primitives_uvp(const CaseConsts<T>& k, T rho, T 
     u = M::div(ru, r);
     v = M::div(rv, r);
     T ke = T(0.5) * r * (u * u + v * v);
-    T eta = (E > T(0)) ? M::div(re, E) : T(1);
-    T e = (eta < T(0.001f)) ? re : E - ke;
+    T e = (E > T(0) && re < T(0.001f) * E) ? re : E - ke;
     p = tmax((T)k.gm1 * e, (T)k.min_p);
 }
 
@@ -195,10 +196,12 @@ __device__ __forceinline__ void phys_flux(T rho, T un, T ut, T p, T E, T f[4], T
 template <class T>
 __device__ __forceinline__ void hllc(const CaseConsts<T>& k, T rL, T uL, T vL, T pL, T EL,
                                      T rR, T uR, T vR, T pR, T ER, T f[4]) {
-    T irL = Num<T>::rcp(rL), irR = Num<T>::rcp(rR);
-    T cL = Num<T>::sqrt((T)k.gamma * pL * irL);
-    T cR = Num<T>::sqrt((T)k.gamma * pR * irR);
-    T sL = Num<T>::sqrt(rL), sR = Num<T>::sqrt(rR);
+    // one reciprocal square root per side gives 1/rho and sqrt(rho)
+    T rsL = Num<T>::rsqrt(rL), rsR = Num<T>::rsqrt(rR);
+    T irL = rsL * rsL, irR = rsR * rsR;
+    T sL = rL * rsL, sR = rR * rsR;
+    T cL = Num<T>::sqrt((T)k.gamma * pL) * rsL;
+    T cR = Num<T>::sqrt((T)k.gamma * pR) * rsR;
     T id = Num<T>::rcp(sL + sR + T(1e-10f));
     T u_roe = (sL * uL + sR * uR) * id;
     T v_roe = (sL * vL + sR * vR) * id;
@@ -319,23 +322,29 @@ __device__ __forceinline__ void face_flux(const CaseConsts<T>& k, const T rho[4]
     out[4] = (uL + uR >= T(0)) ? reL * uL : reR * uR;
 }
 
-// Riemann invariants (solver.cu:680-841) in boundary-normal form.
-template <class T, class M = Num<T>>
-__device__ __forceinline__ void riemann_from_prim(const CaseConsts<T>& k, T rho, T un, T ut, T p, T R[4]) {
-    T c = M::sqrt(M::div((T)k.gamma * p, rho));
-    R[0] = M::exp(-(T)k.gamma * M::log(rho)) * p;
-    R[1] = ut;
-    R[2] = un + c * (T)k.two_over_gm1;
-    R[3] = un - c * (T)k.two_over_gm1;
-}
-template <class T, class M = Num<T>>
-__device__ __forceinline__ void prim_from_riemann(const CaseConsts<T>& k, const T R[4], T& rho, T& un, T& ut, T& p) {
-    ut = R[1];
-    un = (R[2] + R[3]) * T(0.5);
-    T c = (R[2] - R[3]) * (T)k.gm1_over_4;
+// Characteristic (non-reflecting) boundary state.  The reference builds the
+// four Riemann invariants R1 = p rho^-gamma, R2 = u_t, R3/R4 = u_n +- 2c/(gamma-1)
+// of the local and of the far-field state (solver.cu:680-841), takes each from
+// the local state when its characteristic leaves the domain and from the far
+// field otherwise (solver.cu:1925-2030), and inverts them.  The invariants are
+// kept split into their (u_n, c) parts here, so that u_n = (R3 + R4)/2 and
+// c = (R3 - R4)(gamma-1)/4 are formed without cancelling two terms of size
+// 2c/(gamma-1) (in FP32 that cancellation alone leaves u_n with an absolute
+// error near 1e-7 c); the result is the same in exact arithmetic.
+template <class T>
+__device__ __forceinline__ void nrbc_state(const CaseConsts<T>& k, T un_f, T ut_f, T rho_in, T un, T ut, T p,
+                                           bool o1, bool o3, bool o4, T& rho_o, T& un_o, T& ut_o, T& p_o) {
+    typedef Precise<T> M;
+    T c_l = M::sqrt((T)k.gamma * p / rho_in);
+    T s = o1 ? M::exp(-(T)k.gamma * M::log(rho_in)) * p : (T)k.s_far;
+    ut_o = o1 ? ut : ut_f;
+    T un3 = o3 ? un : un_f, c3 = o3 ? c_l : (T)k.c_far;
+    T un4 = o4 ? un : un_f, c4 = o4 ? c_l : (T)k.c_far;
+    un_o = (un3 + un4) * T(0.5) + (c3 - c4) * (T)k.inv_gm1;
+    T c = (un3 - un4) * (T)k.gm1_over_4 + (c3 + c4) * T(0.5);
     T c2 = c * c;
-    rho = M::exp((T)k.inv_gm1 * M::log(M::div(c2 * (T)k.inv_gamma, R[0])));
-    p = rho * c2 * (T)k.inv_gamma;
+    rho_o = M::exp((T)k.inv_gm1 * M::log(c2 * (T)k.inv_gamma / s));
+    p_o = rho_o * c2 * (T)k.inv_gamma;
 }
 
 // Point-wise boundary conditions of applyBoundaryConditionsKernel
@@ -353,18 +362,12 @@ __device__ __forceinline__ bool bc_pointwise(const CaseConsts<T>& k, int i, int 
         T u, vv, p;
         primitives_uvp<T, Precise<T>>(k, v[0], v[1], v[2], v[3], v[4], u, vv, p);
         T cl = Precise<T>::sqrt((T)k.gamma * p / rloc);
-        T R[4];
-        riemann_from_prim<T, Precise<T>>(k, v[0], u, vv, p, R);
         bool o1 = u > T(0), o3 = u + cl > T(0), o4 = u - cl > T(0);
-        R[0] = o1 ? R[0] : (T)k.Rinf_x[0];
-        R[1] = o1 ? R[1] : (T)k.Rinf_x[1];
-        R[2] = o3 ? R[2] : (T)k.Rinf_x[2];
-        R[3] = o4 ? R[3] : (T)k.Rinf_x[3];
-        T rho;
-        prim_from_riemann<T, Precise<T>>(k, R, rho, u, vv, p);
-        v[0] = rho; v[1] = rho * u; v[2] = rho * vv;
-        v[4] = p * (T)k.inv_gm1;
-        v[3] = v[4] + T(0.5) * rho * (u * u + vv * vv);
+        T rho, un, ut, pp;
+        nrbc_state(k, (T)k.u_inf, (T)k.v_inf, v[0], u, vv, p, o1, o3, o4, rho, un, ut, pp);
+        v[0] = rho; v[1] = rho * un; v[2] = rho * ut;
+        v[4] = pp * (T)k.inv_gm1;
+        v[3] = v[4] + T(0.5) * rho * (un * un + ut * ut);
         return true;
     }
     bool touched = false;
@@ -378,16 +381,11 @@ __device__ __forceinline__ bool bc_pointwise(const CaseConsts<T>& k, int i, int 
         bool o1, o3, o4;
         if (side == 0) { o1 = vv < T(0); o3 = vv + cl < T(0); o4 = vv - cl < T(0); }
         else { o1 = vv > T(0); o3 = vv + cl > T(0); o4 = vv - cl > T(0); }
-        T R[4];
-        riemann_from_prim<T, Precise<T>>(k, rloc, vv, u, p, R);
-        R[0] = o1 ? R[0] : (T)k.Rinf_y[0];
-        R[1] = o1 ? R[1] : (T)k.Rinf_y[1];
-        R[2] = o3 ? R[2] : (T)k.Rinf_y[2];
-        R[3] = o4 ? R[3] : (T)k.Rinf_y[3];
-        prim_from_riemann<T, Precise<T>>(k, R, rloc, vv, u, p);
-        v[0] = rloc; v[1] = rloc * u; v[2] = rloc * vv;
-        v[4] = p * (T)k.inv_gm1;
-        v[3] = v[4] + T(0.5) * rloc * (u * u + vv * vv);
+        T rho, un, ut, pp;
+        nrbc_state(k, (T)k.v_inf, (T)k.u_inf, rloc, vv, u, p, o1, o3, o4, rho, un, ut, pp);
+        v[0] = rho; v[1] = rho * ut; v[2] = rho * un;
+        v[4] = pp * (T)k.inv_gm1;
+        v[3] = v[4] + T(0.5) * rho * (ut * ut + un * un);
     }
     if (c == CT_SOLID) {
         v[0] = (T)k.rho_inf; v[1] = T(0); v[2] = T(0); v[3] = (T)k.e_inf; v[4] = (T)k.e_inf;

@@ -138,7 +138,7 @@ int Engine::set_arena(size_t bytes) {
 }
 
 int Engine::set_tile(int tx, int ty) {
-    if (!tile_supported(tx, ty)) return set_error(FVM_ERR_INVALID_ARGUMENT, "unsupported tile shape");
+    if (!tile_supported(tx, ty, fp64_)) return set_error(FVM_ERR_INVALID_ARGUMENT, "unsupported tile shape");
     if (tx == tx_ && ty == ty_) return FVM_OK;
     tx_ = tx;
     ty_ = ty;
@@ -284,15 +284,8 @@ static void fill_consts(CaseConsts<T>& k, const fvm_params& p) {
     k.p_gauge = (sizeof(T) == 4) ? k.p_inf : T(0);
     // far-field Riemann invariants (computeFarfieldRiemannInvarX/Y, solver.cu:799-841)
     T c_inf = std::sqrt(k.gamma * k.p_inf / k.rho_inf);
-    T R0 = std::exp(-k.gamma * std::log(k.rho_inf)) * k.p_inf;
-    k.Rinf_x[0] = R0;
-    k.Rinf_x[1] = k.v_inf;
-    k.Rinf_x[2] = k.u_inf + c_inf * k.two_over_gm1;
-    k.Rinf_x[3] = k.u_inf - c_inf * k.two_over_gm1;
-    k.Rinf_y[0] = R0;
-    k.Rinf_y[1] = k.u_inf;
-    k.Rinf_y[2] = k.v_inf + c_inf * k.two_over_gm1;
-    k.Rinf_y[3] = k.v_inf - c_inf * k.two_over_gm1;
+    k.c_far = c_inf;
+    k.s_far = std::exp(-k.gamma * std::log(k.rho_inf)) * k.p_inf;
     // geometry: the per-cell cosf/sinf of the reference (solver.cu:190-210,
     // 259-260, 330-331, 421-422, 451-452) evaluated once here
     Geom& g = k.geo;

@@ -228,13 +228,14 @@ __global__ __launch_bounds__(NT) void k_bc_inplace(const CaseDesc<T>* __restrict
 template <class T, int TX, int TY, int MODE, bool KAHAN>
 __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
                                               const int* __restrict__ count, Roles ri, Roles rv) {
-    constexpr int SX = TX + 4, SY = TY + 4, CPT = (TX * TY) / NT;
-    static_assert(CPT * NT == TX * TY, "tile must be a multiple of the workgroup size");
+    constexpr int SX = TX + 4, SY = TY + 4, CPT = (TX * TY + NT - 1) / NT;
     constexpr int FX = (TX + 1) * TY, FY = TX * (TY + 1);
     constexpr int FMAX = FX > FY ? FX : FY;
     __shared__ T s_r[SY * SX], s_u[SY * SX], s_v[SY * SX], s_p[SY * SX];
     __shared__ uint8_t s_ct[SY * SX];
-    __shared__ T s_f[5][FMAX];
+    // five flux components + the interface velocity of the -p div(v) source
+    // (mass flux over the mean clamped density of the two cells, solver.cu:1787-1794)
+    __shared__ T s_f[6][FMAX];
     __shared__ float s_red[NT / 64];
 
     const int n_items = *count;
@@ -280,6 +281,7 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
             int fi = n % (TX + 1) - 1, fj = n / (TX + 1);
             int gi = i0 + fi, gj = j0 + fj;
             T o[5] = {T(0), T(0), T(0), T(0), T(0)};
+            T uf = T(0);
             if (gi >= 0 && gi < nx - 1 && gj < ny) {
                 int b = (fj + 2) * SX + (fi + 1);
                 T rr[4], un[4], ut[4], pp[4];
@@ -289,9 +291,11 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
                     rr[m] = s_r[b + m]; un[m] = s_u[b + m]; ut[m] = s_v[b + m]; pp[m] = s_p[b + m]; cc[m] = s_ct[b + m];
                 }
                 face_flux(k, rr, un, ut, pp, cc, o);
+                uf = T(2) * o[0] * Num<T>::rcp(tmax(rr[1], k.min_rho) + tmax(rr[2], k.min_rho));
             }
 #pragma unroll
             for (int c = 0; c < 5; c++) s_f[c][n] = o[c];
+            s_f[5][n] = uf;
         }
         __syncthreads();
 
@@ -300,12 +304,13 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
             int n = threadIdx.x + q * NT;
+            if (n >= TX * TY) break;
             int li = n % TX, lj = n / TX;
             int a = lj * (TX + 1) + li;
 #pragma unroll
             for (int c = 0; c < 5; c++) res[q][c] = (s_f[c][a + 1] - s_f[c][a]) * (T)k.inv_dx;
-            fxr[q] = s_f[0][a + 1];
-            fxl[q] = s_f[0][a];
+            fxr[q] = s_f[5][a + 1];
+            fxl[q] = s_f[5][a];
         }
         __syncthreads();
 
@@ -314,6 +319,7 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
             int fi = n % TX, fj = n / TX - 1;
             int gi = i0 + fi, gj = j0 + fj;
             T o[5] = {T(0), T(0), T(0), T(0), T(0)};
+            T uf = T(0);
             if (gj >= 0 && gj < ny - 1 && gi < nx) {
                 int b = (fj + 1) * SX + (fi + 2);
                 T rr[4], un[4], ut[4], pp[4];
@@ -324,8 +330,10 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
                     rr[m] = s_r[s]; un[m] = s_v[s]; ut[m] = s_u[s]; pp[m] = s_p[s]; cc[m] = s_ct[s];
                 }
                 face_flux(k, rr, un, ut, pp, cc, o);
+                uf = T(2) * o[0] * Num<T>::rcp(tmax(rr[1], k.min_rho) + tmax(rr[2], k.min_rho));
             }
             s_f[0][n] = o[0]; s_f[1][n] = o[2]; s_f[2][n] = o[1]; s_f[3][n] = o[3]; s_f[4][n] = o[4];
+            s_f[5][n] = uf;
         }
         __syncthreads();
 
@@ -333,14 +341,15 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
             int n = threadIdx.x + q * NT;
+            if (n >= TX * TY) break;
             int li = n % TX, lj = n / TX;
             int gi = i0 + li, gj = j0 + lj;
             int a = lj * TX + li;
             T dfy[5];
 #pragma unroll
             for (int c = 0; c < 5; c++) dfy[c] = (s_f[c][a + TX] - s_f[c][a]) * (T)k.inv_dy;
-            fyt[q] = s_f[0][a + TX];
-            fyb[q] = s_f[0][a];
+            fyt[q] = s_f[5][a + TX];
+            fyb[q] = s_f[5][a];
             if (gi >= nx || gj >= ny) continue;
             const size_t g = (size_t)gj * nx + gi;
             const int sc = (lj + 2) * SX + (li + 2);
@@ -355,14 +364,7 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
                 const bool ex = (gi == 0 || gi == nx - 1), ey = (gj == 0 || gj == ny - 1);
                 T src = T(0);
                 if (!ex && !ey) {
-                    T rc = tmax(s_r[sc], (T)k.min_rho);
-                    T rxR = tmax(s_r[sc + 1], (T)k.min_rho), rxL = tmax(s_r[sc - 1], (T)k.min_rho);
-                    T ryT = tmax(s_r[sc + SX], (T)k.min_rho), ryB = tmax(s_r[sc - SX], (T)k.min_rho);
-                    T ufR = T(2) * fxr[q] * Num<T>::rcp(rc + rxR);
-                    T ufL = T(2) * fxl[q] * Num<T>::rcp(rxL + rc);
-                    T vfT = T(2) * fyt[q] * Num<T>::rcp(rc + ryT);
-                    T vfB = T(2) * fyb[q] * Num<T>::rcp(ryB + rc);
-                    T divv = (ufR - ufL) * (T)k.inv_dx + (vfT - vfB) * (T)k.inv_dy;
+                    T divv = (fxr[q] - fxl[q]) * (T)k.inv_dx + (fyt[q] - fyb[q]) * (T)k.inv_dy;
                     src = -s_p[sc] * divv;
                 }
                 T d[5];
@@ -381,15 +383,14 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
                 nr = tmax(nr, (T)k.min_rho);
                 T ir = Num<T>::rcp(nr);
                 T un = nru * ir, vn = nrv * ir;
-                T vm = Num<T>::sqrt(un * un + vn * vn);
-                if (vm > (T)k.vmax) {
-                    T s = (T)k.vmax * Num<T>::rcp(vm);
+                T v2 = un * un + vn * vn;
+                if (v2 > (T)k.vmax * (T)k.vmax) {
+                    T s = (T)k.vmax * Num<T>::rsqrt(v2);
                     un *= s; vn *= s;
                     nru = nr * un; nrv = nr * vn;
                 }
                 T ke = T(0.5) * nr * (un * un + vn * vn);
-                T eta = (nE > T(0)) ? Num<T>::div(nre, nE) : T(1);
-                T e = (eta < T(0.001f)) ? nre : nE - ke;
+                T e = (nE > T(0) && nre < T(0.001f) * nE) ? nre : nE - ke;
                 T rRi = nr * (T)k.R * (T)k.inv_gm1;
                 e = tmax(e, rRi * (T)k.min_T);
                 e = tmin(e, rRi * (T)k.max_T);
@@ -616,8 +617,7 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
 template <class T, int TX, int TY, bool FINAL>
 __global__ __launch_bounds__(NT) void k_diffuse(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
                                                 const int* __restrict__ count, Roles r) {
-    constexpr int SX = TX + 4, SY = TY + 4, QX = TX + 2, QY = TY + 2, CPT = (TX * TY) / NT;
-    static_assert(CPT * NT == TX * TY, "tile must be a multiple of the workgroup size");
+    constexpr int SX = TX + 4, SY = TY + 4, QX = TX + 2, QY = TY + 2, CPT = (TX * TY + NT - 1) / NT;
     __shared__ T s_u[SY * SX], s_v[SY * SX], s_T[SY * SX];
     __shared__ uint8_t s_ct[SY * SX];
     __shared__ T s_txx[QY * QX], s_tyy[QY * QX], s_txy[QY * QX], s_qx[QY * QX], s_qy[QY * QX];
@@ -691,6 +691,7 @@ __global__ __launch_bounds__(NT) void k_diffuse(const CaseDesc<T>* __restrict__ 
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
             int n = threadIdx.x + q * NT;
+            if (n >= TX * TY) break;
             int li = n % TX, lj = n / TX;
             int gi = i0 + li, gj = j0 + lj;
             if (gi >= nx || gj >= ny) continue;
@@ -885,10 +886,14 @@ struct Launch {
     }
 };
 
-#define FVM_TILE_TABLE(X) X(64, 4) X(64, 8) X(32, 8) X(64, 16) X(128, 4) X(128, 8) X(32, 16)
+// Tile shapes compiled for the FP32 kernels (the autotuner chooses among them)
+// and for the FP64 kernels.
+#define FVM_TILE_TABLE(X) X(64, 4) X(64, 6) X(64, 7) X(64, 8) X(64, 12) X(64, 16) X(32, 8) X(32, 16) X(128, 4) X(128, 8) X(96, 8)
+#define FVM_TILE_TABLE64(X) X(64, 8) X(32, 8)
 
 template <class T>
-static bool dispatch_tile(int tx, int ty, const KernelCall& kc) {
+static bool dispatch_tile(int tx, int ty, const KernelCall& kc);
+
 #define FVM_TRY(A, B)                                                                              \
     if (tx == A && ty == B) {                                                                      \
         switch (kc.op) {                                                                           \
@@ -902,18 +907,34 @@ static bool dispatch_tile(int tx, int ty, const KernelCall& kc) {
         }                                                                                          \
         return true;                                                                               \
     }
+
+template <>
+bool dispatch_tile<float>(int tx, int ty, const KernelCall& kc) {
+    using T = float;
     FVM_TILE_TABLE(FVM_TRY)
-#undef FVM_TRY
     return false;
 }
+
+template <>
+bool dispatch_tile<double>(int tx, int ty, const KernelCall& kc) {
+    using T = double;
+    FVM_TILE_TABLE64(FVM_TRY)
+    return false;
+}
+#undef FVM_TRY
+
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc) {
     return fp64 ? dispatch_tile<double>(tx, ty, kc) : dispatch_tile<float>(tx, ty, kc);
 }
 
-bool tile_supported(int tx, int ty) {
+bool tile_supported(int tx, int ty, bool fp64) {
 #define FVM_HAS(A, B) if (tx == A && ty == B) return true;
-    FVM_TILE_TABLE(FVM_HAS)
+    if (fp64) {
+        FVM_TILE_TABLE64(FVM_HAS)
+    } else {
+        FVM_TILE_TABLE(FVM_HAS)
+    }
 #undef FVM_HAS
     return false;
 }

@@ -20,7 +20,7 @@ struct KernelCall {
 };
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc);
-bool tile_supported(int tx, int ty);
+bool tile_supported(int tx, int ty, bool fp64);
 void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
                   const int* n, Roles ri, Roles rv);
 void launch_ghost_prep(bool fp64, hipStream_t s, const void* c, const WorkItem* items, int n);

@@ -88,5 +88,8 @@ class BatchEngine:
     def synchronize(self):
         check(lib().fvm_batch_synchronize(self._h), "fvm_batch_synchronize")
 
+    def set_tile(self, tx: int, ty: int):
+        check(lib().fvm_batch_set_tile(self._h, tx, ty), "fvm_batch_set_tile")
+
     def use_graph(self, on: bool):
         check(lib().fvm_batch_use_graph(self._h, 1 if on else 0), "fvm_batch_use_graph")

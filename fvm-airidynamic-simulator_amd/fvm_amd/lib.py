@@ -81,6 +81,7 @@ SIGNATURES = [
     ("fvm_batch_active_cases", _i, [_vp, _i32p]),
     ("fvm_batch_bytes_in_use", _i, [_vp, _szp]),
     ("fvm_batch_synchronize", _i, [_vp]),
+    ("fvm_batch_set_tile", _i, [_vp, _i, _i]),
     ("fvm_batch_use_graph", _i, [_vp, _i]),
 ]
 

@@ -198,7 +198,9 @@ int fvm_gpu_memory_info(int device, size_t* free_bytes, size_t* total_bytes);
 int fvm_synchronize(fvm_solver* s);
 /* Grid size (getNx / getNy, solver.cuh:62-63). */
 int fvm_get_size(fvm_solver* s, int* nx, int* ny);
-/* Tile shape of the stage kernel: (64,8) (64,16) (32,8) (128,4) ...; 0,0 = auto. */
+/* Tile shape of the stage and viscous kernels, one of the compiled shapes
+ * (64x4, 64x6, 64x7, 64x8, 64x12, 64x16, 32x8, 32x16, 128x4, 128x8, 96x8;
+ * FP64: 64x8, 32x8); 0,0 = the autotuned default for the grid. */
 int fvm_set_tile(fvm_solver* s, int tile_x, int tile_y);
 /* Name and average duration of each kernel launched by the last fvm_advance
  * call when timing is enabled (HIP events on the handle's stream). */
@@ -234,6 +236,8 @@ int fvm_batch_get_cell_types(fvm_batch* b, int32_t case_id, uint8_t* host);
 int fvm_batch_active_cases(fvm_batch* b, int32_t* n);
 int fvm_batch_bytes_in_use(fvm_batch* b, size_t* bytes);
 int fvm_batch_synchronize(fvm_batch* b);
+/* Tile shape of the batched kernels (0,0 = the autotuned default). */
+int fvm_batch_set_tile(fvm_batch* b, int tile_x, int tile_y);
 /* Use a captured HIP graph for fvm_batch_advance (default on). */
 int fvm_batch_use_graph(fvm_batch* b, int enable);
 
