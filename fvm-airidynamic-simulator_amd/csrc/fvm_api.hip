@@ -356,6 +356,12 @@ int fvm_conservation_sums(fvm_solver* s, double out4[4]) {
     return s->eng->conservation(s->case_id, out4);
 }
 
+int fvm_forces(fvm_solver* s, double out3[3]) {
+    CHECK_SOLVER(s);
+    if (!out3) return set_error(FVM_ERR_INVALID_ARGUMENT, "null buffer");
+    return s->eng->forces(s->case_id, out3);
+}
+
 int fvm_get_time(fvm_solver* s, double* t, int64_t* steps) {
     CHECK_SOLVER(s);
     long long n = 0;
@@ -433,6 +439,8 @@ int fvm_enable_timing(fvm_solver* s, int enable) {
 
 int fvm_kernel_timing(fvm_solver* s, int index, char* name, int name_len, double* avg_ms, int64_t* launches) {
     CHECK_SOLVER(s);
+    int rc = s->eng->flush_timings();
+    if (rc) return rc;
     const auto& t = s->eng->timings();
     if (index < 0 || index >= (int)t.size()) return set_error(FVM_ERR_NOT_FOUND, "no such timing entry");
     if (name && name_len > 0) {
@@ -540,6 +548,11 @@ int fvm_batch_get_state(fvm_batch* b, int32_t case_id, float* host5) {
 int fvm_batch_get_cell_types(fvm_batch* b, int32_t case_id, uint8_t* host) {
     if (!b || !host) return set_error(FVM_ERR_INVALID_ARGUMENT, "null argument");
     return b->eng->read_cell_types(case_id, host);
+}
+
+int fvm_batch_forces(fvm_batch* b, int32_t case_id, double out3[3]) {
+    if (!b || !out3) return set_error(FVM_ERR_INVALID_ARGUMENT, "null argument");
+    return b->eng->forces(case_id, out3);
 }
 
 int fvm_batch_active_cases(fvm_batch* b, int32_t* n) {

@@ -108,8 +108,9 @@ Engine::~Engine() {
     if (d_stat_) (void)hipFree(d_stat_);
     if (d_cons_) (void)hipFree(d_cons_);
     if (h_pinned_) (void)hipHostFree(h_pinned_);
-    if (ev0_) (void)hipEventDestroy(ev0_);
-    if (ev1_) (void)hipEventDestroy(ev1_);
+    for (auto& ts : tset_)
+        for (auto& ev : ts.ev)
+            if (ev) (void)hipEventDestroy(ev);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -123,8 +124,9 @@ int Engine::init_stream() {
     if (!d_stat_) FVM_HIP(hipMalloc((void**)&d_stat_, 16 * sizeof(unsigned)));
     if (!d_cons_) FVM_HIP(hipMalloc((void**)&d_cons_, (1024 * 4 + 8) * sizeof(double)));
     if (!h_pinned_) FVM_HIP(hipHostMalloc((void**)&h_pinned_, 16 * sizeof(float)));
-    if (!ev0_) FVM_HIP(hipEventCreate(&ev0_));
-    if (!ev1_) FVM_HIP(hipEventCreate(&ev1_));
+    for (auto& ts : tset_)
+        for (auto& ev : ts.ev)
+            if (!ev) FVM_HIP(hipEventCreate(&ev));
     return FVM_OK;
 }
 
@@ -707,22 +709,62 @@ void Engine::drop_graphs() {
     graph_valid_ = false;
 }
 
+int Engine::flush_timed_set(int set) {
+    TimedSet& ts = tset_[set];
+    if (!ts.pending) return FVM_OK;
+    ts.pending = false;
+    FVM_HIP(hipEventSynchronize(ts.ev[ts.n]));
+    for (int i = 0; i < ts.n; i++) {
+        float ms = 0;
+        FVM_HIP(hipEventElapsedTime(&ms, ts.ev[i], ts.ev[i + 1]));
+        bool found = false;
+        for (auto& kv : timing_list_)
+            if (kv.first == ts.name[i]) {
+                kv.second.total_ms += ms;
+                kv.second.launches++;
+                found = true;
+                break;
+            }
+        if (!found) {
+            KernelStat st;
+            st.total_ms = ms;
+            st.launches = 1;
+            timing_list_.push_back({ts.name[i], st});
+        }
+    }
+    return FVM_OK;
+}
+
+int Engine::flush_timings() {
+    int rc = flush_timed_set(tcur_ ^ 1);
+    if (!rc) rc = flush_timed_set(tcur_);
+    return rc;
+}
+
+void Engine::clear_timings() {
+    tset_[0].pending = tset_[1].pending = false;
+    timing_list_.clear();
+}
+
+// Opens the event set of one step: the set used two steps ago is read first.
+int Engine::timed_begin() {
+    if (!timing_) return FVM_OK;
+    int rc = flush_timed_set(tcur_);
+    if (rc) return rc;
+    TimedSet& ts = tset_[tcur_];
+    ts.n = 0;
+    FVM_HIP(hipEventRecord(ts.ev[0], stream_));
+    return FVM_OK;
+}
+
+// Closes the interval of the kernel just launched.
 int Engine::timed(const char* name, int rc_before) {
     if (!timing_ || rc_before) return rc_before;
-    FVM_HIP(hipEventRecord(ev1_, stream_));
-    FVM_HIP(hipEventSynchronize(ev1_));
-    float ms = 0;
-    FVM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-    for (auto& kv : timing_list_)
-        if (kv.first == name) {
-            kv.second.total_ms += ms;
-            kv.second.launches++;
-            return FVM_OK;
-        }
-    KernelStat st;
-    st.total_ms = ms;
-    st.launches = 1;
-    timing_list_.push_back({name, st});
+    TimedSet& ts = tset_[tcur_];
+    if (ts.n >= kTimedMax) return set_error(FVM_ERR_INVALID_ARGUMENT, "too many timed launches in one step");
+    ts.name[ts.n] = name;
+    FVM_HIP(hipEventRecord(ts.ev[ts.n + 1], stream_));
+    ts.n++;
     return FVM_OK;
 }
 
@@ -745,12 +787,9 @@ int Engine::enqueue_step(int ph) {
     kc.stream = stream_;
     kc.cases = d_cases_;
     kc.kahan = kahan_;
-    int rc = 0;
-    auto mark = [&]() {
-        if (timing_) (void)hipEventRecord(ev0_, stream_);
-    };
+    int rc = timed_begin();
+    if (rc) return rc;
     if (any_visc_) {
-        mark();
         kc.op = OP_DIFFUSE;
         kc.grid = dim3(grid_visc_);
         kc.work = d_tiles_visc_;
@@ -760,7 +799,6 @@ int Engine::enqueue_step(int ph) {
         launch_tiled(fp64_, tx_, ty_, kc);
         rc = timed("diffuse_half1", rc);
     }
-    mark();
     kc.op = OP_STAGE1;
     kc.grid = dim3(grid_tiles_);
     kc.work = d_tiles_all_;
@@ -769,17 +807,14 @@ int Engine::enqueue_step(int ph) {
     kc.rv = v_s1;
     launch_tiled(fp64_, tx_, ty_, kc);
     rc = timed("stage1", rc);
-    mark();
     launch_ghost(fp64_, false, dim3(grid_ghosts_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g1, v_g1);
     rc = timed("ghost1", rc);
-    mark();
     kc.op = OP_STAGE2;
     kc.ri = i_s2;
     kc.rv = v_s2;
     launch_tiled(fp64_, tx_, ty_, kc);
     rc = timed("stage2", rc);
     if (any_visc_) {
-        mark();
         kc.op = OP_DIFFUSE_FINAL;
         kc.grid = dim3(grid_visc_);
         kc.work = d_tiles_visc_;
@@ -789,10 +824,13 @@ int Engine::enqueue_step(int ph) {
         launch_tiled(fp64_, tx_, ty_, kc);
         rc = timed("diffuse_half2", rc);
     }
-    mark();
     launch_ghost(fp64_, true, dim3(grid_ghosts_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g2, v_g2);
     rc = timed("ghost2", rc);
     if (rc) return rc;
+    if (timing_) {
+        tset_[tcur_].pending = true;
+        tcur_ ^= 1;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return FVM_OK;
@@ -865,7 +903,7 @@ int Engine::advance(int nsteps, bool use_graph) {
 
 int Engine::sync() {
     FVM_HIP(hipStreamSynchronize(stream_));
-    return FVM_OK;
+    return flush_timings();
 }
 
 int Engine::read_state(int id, void* host, bool as_f64) {
@@ -1025,6 +1063,18 @@ int Engine::conservation(int id, double out[4]) {
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(stream_));
     FVM_HIP(hipMemcpy(out, d_cons_, 4 * sizeof(double), hipMemcpyDeviceToHost));
+    return FVM_OK;
+}
+
+int Engine::forces(int id, double out[3]) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    launch_forces(fp64_, stream_, d_cases_, id, current_set(*hc), d_cons_ + 8, 1024, d_cons_);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(stream_));
+    double r[4];
+    FVM_HIP(hipMemcpy(r, d_cons_, 4 * sizeof(double), hipMemcpyDeviceToHost));
+    out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
     return FVM_OK;
 }
 

@@ -106,6 +106,7 @@ public:
     int field(int id, int field, float* dev_out);
     int stat(int id, int mode, float* out);
     int conservation(int id, double out[4]);
+    int forces(int id, double out[3]);   // Fx, Fy, number of wall faces
 
     int active_count() const;
     int case_slots() const { return (int)cases_.size(); }
@@ -118,8 +119,12 @@ public:
     hipStream_t stream() const { return stream_; }
 
     void enable_timing(bool on) { timing_ = on; }
+    // Per-kernel durations: events recorded between the launches of a step,
+    // with no host wait inside the step; the events of a step are read two
+    // steps later (or at flush), so the queue never drains while timing.
+    int flush_timings();
     const std::vector<std::pair<std::string, KernelStat>>& timings() const { return timing_list_; }
-    void clear_timings() { timing_list_.clear(); }
+    void clear_timings();
 
 private:
     int build_consts(HostCase& hc, const fvm_params& p, void* desc_out);
@@ -129,7 +134,9 @@ private:
     int enqueue_step(int phase);
     int capture_graphs();
     void drop_graphs();
+    int timed_begin();
     int timed(const char* name, int rc_before);
+    int flush_timed_set(int set);
 
     int device_;
     bool fp64_, kahan_;
@@ -161,7 +168,15 @@ private:
     bool use_arena_ = false;
 
     bool timing_ = false;
-    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    static constexpr int kTimedMax = 8;
+    struct TimedSet {
+        hipEvent_t ev[kTimedMax + 1] = {};
+        const char* name[kTimedMax] = {};
+        int n = 0;
+        bool pending = false;
+    };
+    TimedSet tset_[2];
+    int tcur_ = 0;
     std::vector<std::pair<std::string, KernelStat>> timing_list_;
 
     unsigned* d_stat_ = nullptr;

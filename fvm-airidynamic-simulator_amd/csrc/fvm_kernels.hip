@@ -854,6 +854,62 @@ __global__ __launch_bounds__(NT) void k_cons_partial(const CaseDesc<T>* __restri
         for (int f = 0; f < 4; f++) partial[blockIdx.x * 4 + f] = s[f][0];
 }
 
+// Pressure force on the obstacle, per unit depth.  computeFluxesKernel gives
+// every face with a solid or ghost cell on either side the wall flux
+// (0, p, 0, 0, 0) with p of the face's left (x) or lower (y) cell
+// (solver.cu:1589-1598, 1673-1682).  The fluid cell on the other side of such a
+// face loses that normal momentum (wall to its right/top) or gains it (wall to
+// its left/bottom); the sum over all wall faces next to cells whose residual in
+// that direction is applied (not on the domain edge, solver.cu:1744-1760) is the
+// momentum the body takes from the fluid per unit time.  acc = (Fx, Fy, faces, 0).
+template <class T>
+__global__ __launch_bounds__(NT) void k_force_partial(const CaseDesc<T>* __restrict__ cases, int case_id, int set,
+                                                      double* partial) {
+    __shared__ double s[4][NT];
+    const CaseDesc<T>& cd = cases[case_id];
+    const CaseConsts<T>& k = cd.c;
+    const int nx = k.nx, ny = k.ny;
+    const size_t n = (size_t)nx * ny;
+    const T* U0 = (const T*)cd.b.U[set][0];
+    const T* U1 = (const T*)cd.b.U[set][1];
+    const T* U2 = (const T*)cd.b.U[set][2];
+    const T* U3 = (const T*)cd.b.U[set][3];
+    const T* U4 = (const T*)cd.b.U[set][4];
+    const uint8_t* ct = cd.b.ct;
+    auto wall = [](uint8_t c) { return c == CT_SOLID || c == CT_GHOST; };
+    double acc[4] = {0, 0, 0, 0};
+    for (size_t g = (size_t)blockIdx.x * NT + threadIdx.x; g < n; g += (size_t)gridDim.x * NT) {
+        const int i = (int)(g % nx), j = (int)(g / nx);
+        const bool w = wall(ct[g]);
+        int dir = 0;   // bit 0: x face i+1/2 is a wall face next to an updated cell; bit 1: y face j+1/2
+        double sx = 0, sy = 0;
+        if (i < nx - 1 && w != wall(ct[g + 1])) {
+            const int fi = w ? i + 1 : i;
+            if (fi != 0 && fi != nx - 1) { dir |= 1; sx = w ? -1.0 : 1.0; }
+        }
+        if (j < ny - 1 && w != wall(ct[g + nx])) {
+            const int fj = w ? j + 1 : j;
+            if (fj != 0 && fj != ny - 1) { dir |= 2; sy = w ? -1.0 : 1.0; }
+        }
+        if (dir) {
+            T u, v, p;
+            primitives_uvp<T, Precise<T>>(k, U0[g], U1[g], U2[g], U3[g], U4[g], u, v, p);
+            acc[0] += sx * (double)p * (double)k.dy;
+            acc[1] += sy * (double)p * (double)k.dx;
+            acc[2] += (double)((dir & 1) + (dir >> 1));
+        }
+    }
+    for (int f = 0; f < 4; f++) s[f][threadIdx.x] = acc[f];
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int f = 0; f < 4; f++) s[f][threadIdx.x] += s[f][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int f = 0; f < 4; f++) partial[blockIdx.x * 4 + f] = s[f][0];
+}
+
 __global__ __launch_bounds__(64) void k_cons_final(const double* partial, int nblocks, double* out) {
     if (threadIdx.x < 4) {
         double a = 0;
@@ -994,6 +1050,12 @@ void launch_stat(bool fp64, hipStream_t s, const void* c, int case_id, size_t nc
 void launch_cons(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out) {
     if (fp64) k_cons_partial<double><<<nblocks, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set, partial);
     else k_cons_partial<float><<<nblocks, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, partial);
+    k_cons_final<<<1, 64, 0, s>>>(partial, nblocks, out);
+}
+
+void launch_forces(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out) {
+    if (fp64) k_force_partial<double><<<nblocks, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set, partial);
+    else k_force_partial<float><<<nblocks, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, partial);
     k_cons_final<<<1, 64, 0, s>>>(partial, nblocks, out);
 }
 

@@ -31,5 +31,6 @@ void launch_zero_u32(hipStream_t s, unsigned* p, int n);
 void launch_field(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int field, float* out);
 void launch_stat(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int mode, unsigned* target);
 void launch_cons(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out);
+void launch_forces(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out);
 
 }  // namespace fvm

@@ -68,6 +68,12 @@ class BatchEngine:
               "fvm_batch_get_state")
         return out
 
+    def forces(self, cid: int) -> np.ndarray:
+        out = np.zeros(3, dtype=np.float64)
+        check(lib().fvm_batch_forces(self._h, cid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              "fvm_batch_forces")
+        return out
+
     def cell_types(self, cid: int) -> np.ndarray:
         ny, nx = self._shape[cid]
         out = np.empty((ny, nx), dtype=np.uint8)

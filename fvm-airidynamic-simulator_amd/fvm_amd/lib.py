@@ -59,6 +59,7 @@ SIGNATURES = [
     ("fvm_set_state", _i, [_vp, _fp]),
     ("fvm_get_state_f64", _i, [_vp, _dp]),
     ("fvm_conservation_sums", _i, [_vp, _dp]),
+    ("fvm_forces", _i, [_vp, _dp]),
     ("fvm_get_time", _i, [_vp, _dp, _i64p]),
     ("fvm_max_temperature", _i, [_vp, _fp]),
     ("fvm_max_mach", _i, [_vp, _fp]),
@@ -78,6 +79,7 @@ SIGNATURES = [
     ("fvm_batch_case_status", _i, [_vp, ctypes.c_int32, _i32p, _dp, _i32p]),
     ("fvm_batch_get_state", _i, [_vp, ctypes.c_int32, _fp]),
     ("fvm_batch_get_cell_types", _i, [_vp, ctypes.c_int32, _u8p]),
+    ("fvm_batch_forces", _i, [_vp, ctypes.c_int32, _dp]),
     ("fvm_batch_active_cases", _i, [_vp, _i32p]),
     ("fvm_batch_bytes_in_use", _i, [_vp, _szp]),
     ("fvm_batch_synchronize", _i, [_vp]),

@@ -146,6 +146,12 @@ class CFDSolver:
         check(lib().fvm_conservation_sums(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "fvm_conservation_sums")
         return out
 
+    def forces(self) -> np.ndarray:
+        """Pressure force on the obstacle per unit depth: (Fx, Fy, wall faces)."""
+        out = np.zeros(3, dtype=np.float64)
+        check(lib().fvm_forces(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "fvm_forces")
+        return out
+
     def time(self):
         t = ctypes.c_double()
         n = ctypes.c_int64()

@@ -182,6 +182,14 @@ int fvm_set_state(fvm_solver* s, const float* host5);
 int fvm_get_state_f64(fvm_solver* s, double* host5);
 /* Domain totals of (rho, rho_u, rho_v, E) summed in FP64 on the device. */
 int fvm_conservation_sums(fvm_solver* s, double out4[4]);
+/* Pressure force on the obstacle per unit depth, (Fx, Fy), summed in FP64 over
+ * the wall faces of the current state: the wall flux (0, p, 0, 0, 0) that
+ * computeFluxesKernel (solver.cu:1589-1598, 1673-1682) gives a face with a
+ * solid or ghost cell on either side, counted where the fluid cell on the
+ * other side applies it.  out3[2] = number of such faces.  No reference
+ * counterpart (the reference reports only max T and Mach): it is the drag /
+ * lift diagnostic for long-run comparisons. */
+int fvm_forces(fvm_solver* s, double out3[3]);
 /* Device-side simulated time (FP64 sum of the dt of every step since reset)
  * and step count. */
 int fvm_get_time(fvm_solver* s, double* t, int64_t* steps);
@@ -233,6 +241,8 @@ int fvm_batch_advance(fvm_batch* b, int32_t nsteps);
 int fvm_batch_case_status(fvm_batch* b, int32_t case_id, int32_t* steps_done, double* t, int32_t* finished);
 int fvm_batch_get_state(fvm_batch* b, int32_t case_id, float* host5);
 int fvm_batch_get_cell_types(fvm_batch* b, int32_t case_id, uint8_t* host);
+/* fvm_forces for one case of the batch. */
+int fvm_batch_forces(fvm_batch* b, int32_t case_id, double out3[3]);
 int fvm_batch_active_cases(fvm_batch* b, int32_t* n);
 int fvm_batch_bytes_in_use(fvm_batch* b, size_t* bytes);
 int fvm_batch_synchronize(fvm_batch* b);

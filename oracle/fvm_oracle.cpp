@@ -992,6 +992,41 @@ struct Oracle {
         max_speed = ms; max_nu = mn;
     }
 
+    /* Pressure force on the obstacle (per unit depth) implied by the wall
+     * fluxes of computeFluxesKernel (solver.cu:1589-1598 x faces, 1673-1682
+     * y faces): a face with a solid/ghost cell on either side carries the
+     * normal momentum flux p of its left (lower) cell, and updateKernel
+     * applies it to the cell on the other side unless that cell's residual in
+     * this direction is zeroed on the domain edge (1744-1760).  Momentum leaving
+     * the fluid through such faces is momentum given to the body.
+     * out = (Fx, Fy, number of wall faces counted). */
+    void forces(double out[3]) {
+        compute_prims(U);
+        auto wall = [&](size_t n) { return ct[n] == FVM_CELL_SOLID || ct[n] == FVM_CELL_GHOST; };
+        double fx = 0, fy = 0, nf = 0;
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i + 1 < nx; i++) {           // face between (i,j) and (i+1,j)
+                size_t L = (size_t)j * nx + i, R = L + 1;
+                if (wall(L) == wall(R)) continue;
+                int fluid_i = wall(L) ? i + 1 : i;
+                if (fluid_i == 0 || fluid_i == nx - 1) continue;
+                // the fluid cell L sends +p_L dy into the wall on its right;
+                // the fluid cell R receives p_L dy from the wall on its left
+                fx += (wall(L) ? -1.0 : 1.0) * (double)pp[L] * (double)P.dy;
+                nf += 1;
+            }
+        for (int j = 0; j + 1 < ny; j++)
+            for (int i = 0; i < nx; i++) {               // face between (i,j) and (i,j+1)
+                size_t B = (size_t)j * nx + i, Tp = B + nx;
+                if (wall(B) == wall(Tp)) continue;
+                int fluid_j = wall(B) ? j + 1 : j;
+                if (fluid_j == 0 || fluid_j == ny - 1) continue;
+                fy += (wall(B) ? -1.0 : 1.0) * (double)pp[B] * (double)P.dx;
+                nf += 1;
+            }
+        out[0] = fx; out[1] = fy; out[2] = nf;
+    }
+
     /* readTimeStepResult, solver.cu:3250-3265, evaluated in T */
     T stable_dt(const fvm_params& p) {
         T ms, mn;
@@ -1143,6 +1178,16 @@ void oracle_conservation_sums(fvm_oracle* o, double out[4]) {
         for (size_t i = 0; i < n; i++) s += o->fp64 ? o->d->U.f[c][i] : (double)o->f->U.f[c][i];
         out[c] = s;
     }
+}
+
+void oracle_get_time(fvm_oracle* o, double* t, long long* n) {
+    if (o->fp64) { *t = o->d->t; *n = o->d->nsteps; }
+    else { *t = o->f->t; *n = o->f->nsteps; }
+}
+
+void oracle_forces(fvm_oracle* o, double out[3]) {
+    if (o->fp64) o->d->forces(out);
+    else o->f->forces(out);
 }
 
 /* -------- unit hooks for known-answer tests -------- */

@@ -55,6 +55,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_stable_dt.argtypes = [ctypes.c_void_p, P]
         L.oracle_stable_dt.restype = ctypes.c_double
         L.oracle_conservation_sums.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_forces.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_get_time.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_set_threads.restype = ctypes.c_int
         L.oracle_sdf_point.argtypes = [P, ctypes.c_float, ctypes.c_float]
@@ -149,6 +151,19 @@ class Oracle:
     def conservation_sums(self) -> np.ndarray:
         out = np.zeros(4, dtype=np.float64)
         lib().oracle_conservation_sums(self._h, _dptr(out))
+        return out
+
+    def time(self):
+        """(simulated time, steps) since reset."""
+        t = np.zeros(1, dtype=np.float64)
+        n = ctypes.c_longlong()
+        lib().oracle_get_time(self._h, _dptr(t), ctypes.byref(n))
+        return float(t[0]), int(n.value)
+
+    def forces(self) -> np.ndarray:
+        """Pressure force on the obstacle per unit depth: (Fx, Fy, wall faces)."""
+        out = np.zeros(3, dtype=np.float64)
+        lib().oracle_forces(self._h, _dptr(out))
         return out
 
 

@@ -14,6 +14,7 @@ import pytest
 import oracle as O
 from fvm_amd import params as P
 from exact_riemann import sample
+from states import hydrostatic_state
 
 
 def sod_params(nx, ny, length, along_y=False):
@@ -295,3 +296,36 @@ def test_noise_generator():
     assert abs(u.mean() - 0.5) < 0.01
     assert O.noise_unit(7, 3) == O.noise_unit(7, 3)
     assert O.noise_unit(7, 3) != O.noise_unit(8, 3)
+
+
+@pytest.mark.parametrize("shape", [0, 2, 4])
+@pytest.mark.parametrize("along_y", [False, True])
+def test_forces_discrete_buoyancy(shape, along_y):
+    """A linear pressure field p = 1 + b x on fluid at rest: the wall-flux force
+    is the discrete buoyancy -b * (solid + ghost cells) * dx * dy along the
+    gradient and exactly 0 across it (each row of a one-run body contributes
+    p(first fluid cell) - p(last wall cell) = -b * run length * dx)."""
+    p = P.euler_bow_shock(nx=96, ny=64, radius=10.0, center=(40.0, 32.0))
+    p.obstacle_shape = shape
+    p.obstacle_rotation = 0.3
+    P.compute_derived(p)
+    b = 0.01
+    o = O.Oracle(p, fp64=True)
+    o.set_state(hydrostatic_state(p, b, along_y))
+    f = o.forces()
+    ct = o.cell_types()
+    nwall = int(np.count_nonzero((ct == P.CELL_SOLID) | (ct == P.CELL_GHOST)))
+    expect = -b * nwall * p.dx * p.dy
+    along, across = (f[1], f[0]) if along_y else (f[0], f[1])
+    assert along == pytest.approx(expect, rel=1e-12)
+    assert abs(across) < 1e-12
+    assert f[2] > 0
+    o.close()
+
+
+def test_forces_zero_at_uniform_pressure():
+    p = P.viscous_cylinder(nx=128, ny=64, diameter=16.0, reynolds=100.0)
+    o = O.Oracle(p, fp64=True)
+    f = o.forces()
+    assert abs(f[0]) < 1e-12 and abs(f[1]) < 1e-12
+    o.close()
