@@ -64,6 +64,7 @@ struct CaseBufs {
     float* sdf;
     void* ghosts;         // GhostInfo<T>[n_ghosts]
     unsigned* slots;      // [2 slots][2]: bits of max wave speed, max mu/rho
+    float* tile_max;      // [2][tiles]: per-tile max wave speed, max mu/rho of the step's final kernel
     double* t;            // simulated time
     long long* steps;     // steps taken
 };
@@ -434,6 +435,30 @@ __device__ __forceinline__ void slot_max(unsigned* target, float m) {
     if (!(m > 0.0f)) return;
     unsigned cur = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (__float_as_uint(m) > cur) atomicMax(target, __float_as_uint(m));
+}
+
+// Workgroup maxima of NQ (<= 2) quantities stored for one tile (plain stores,
+// no atomics): tile_max[tile] and tile_max[ntiles + tile].  The step's last
+// kernel reduces a case's tiles into its CFL slot.  Called by the whole
+// workgroup; lds holds NQ x (NT / 64) floats.
+template <int NT, int NQ>
+__device__ __forceinline__ void block_max_store(const float (&v)[NQ], float (*lds)[NT / 64], float* tile_max,
+                                                int ntiles, int tile) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q < NQ; q++) {
+        float m = v[q];
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if (lane == 0) lds[q][wave] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 2; q++) {
+            float m = 0.0f;
+            if (q < NQ)
+                for (int w = 0; w < NT / 64; w++) m = fmaxf(m, lds[q][w]);
+            tile_max[q * ntiles + tile] = m;
+        }
+    }
 }
 
 template <int NT>

@@ -104,6 +104,7 @@ Engine::~Engine() {
     if (d_tiles_all_) (void)hipFree(d_tiles_all_);
     if (d_tiles_visc_) (void)hipFree(d_tiles_visc_);
     if (d_ghosts_) (void)hipFree(d_ghosts_);
+    if (d_case_list_) (void)hipFree(d_case_list_);
     if (d_counts_) (void)hipFree(d_counts_);
     if (d_stat_) (void)hipFree(d_stat_);
     if (d_cons_) (void)hipFree(d_cons_);
@@ -195,7 +196,7 @@ size_t Engine::bytes_in_use() const {
 
 // Plane layout of one case inside its allocation.
 struct CaseLayout {
-    size_t n, plane, sets_off, k_off, ct_off, sdf_off, slots_off, t_off, steps_off, total;
+    size_t n, plane, sets_off, k_off, ct_off, sdf_off, slots_off, t_off, steps_off, tmax_off, total;
 };
 
 static CaseLayout layout_for(const fvm_params& p, bool fp64, bool kahan, int nsets) {
@@ -217,6 +218,10 @@ static CaseLayout layout_for(const fvm_params& p, bool fp64, bool kahan, int nse
     off += 256;
     L.steps_off = off;
     off += 256;
+    // per-tile CFL maxima: every compiled tile shape has tile_x >= 32 and
+    // tile_y >= 4, so ceil(nx/32) * ceil(ny/4) bounds the tile count
+    L.tmax_off = off;
+    off += round256((size_t)((p.nx + 31) / 32) * ((p.ny + 3) / 4) * 2 * sizeof(float));
     L.total = off;
     return L;
 }
@@ -339,6 +344,7 @@ int Engine::upload_desc(HostCase& hc) {
     b.slots = (unsigned*)(base + L.slots_off);
     b.t = (double*)(base + L.t_off);
     b.steps = (long long*)(base + L.steps_off);
+    b.tile_max = (float*)(base + L.tmax_off);
     if (fp64_) {
         CaseDesc<double>* d = (CaseDesc<double>*)hd;
         fill_consts(d->c, hc.p);
@@ -658,9 +664,9 @@ int Engine::set_host_dt(int id, float dt) {
 }
 
 int Engine::rebuild_lists() {
-    std::vector<WorkItem> all, visc, ghosts, oall, ovisc;
+    std::vector<WorkItem> all, visc, ghosts, oall, ovisc, clist;
     const bool had_visc = any_visc_;
-    WorkItem* const old_ptrs[3] = {d_tiles_all_, d_tiles_visc_, d_ghosts_};
+    WorkItem* const old_ptrs[4] = {d_tiles_all_, d_tiles_visc_, d_ghosts_, d_case_list_};
     any_visc_ = false;
     for (HostCase* c : cases_) {
         if (!c || !c->active || c->finished) continue;
@@ -671,12 +677,14 @@ int Engine::rebuild_lists() {
         }
         if (c->p.enable_viscosity) any_visc_ = true;
         for (int g = 0; g < (int)c->ghosts.size(); g++) ghosts.push_back(WorkItem{c->id, g});
+        clist.push_back(WorkItem{c->id, 0});
     }
     xcd_order(all, oall);
     xcd_order(visc, ovisc);
     int rc = upload_items(oall, &d_tiles_all_, &cap_tiles_, d_counts_, 0, stream_);
     if (!rc) rc = upload_items(ovisc, &d_tiles_visc_, &cap_visc_, d_counts_, 1, stream_);
     if (!rc) rc = upload_items(ghosts, &d_ghosts_, &cap_ghosts_, d_counts_, 2, stream_);
+    if (!rc) rc = upload_items(clist, &d_case_list_, &cap_case_list_, d_counts_, 3, stream_);
     if (rc) return rc;
     n_tiles_ = (int)oall.size();
     n_visc_ = (int)ovisc.size();
@@ -687,13 +695,15 @@ int Engine::rebuild_lists() {
         grid_tiles_ = persistent_grid_;
         grid_visc_ = persistent_grid_;
         grid_ghosts_ = 512;
+        grid_reduce_ = 256;
         if (had_visc != any_visc_ || old_ptrs[0] != d_tiles_all_ || old_ptrs[1] != d_tiles_visc_ ||
-            old_ptrs[2] != d_ghosts_)
+            old_ptrs[2] != d_ghosts_ || old_ptrs[3] != d_case_list_)
             drop_graphs();
     } else {
         grid_tiles_ = std::max(1, n_tiles_);
         grid_visc_ = std::max(1, n_visc_);
         grid_ghosts_ = std::max(1, (n_ghosts_ + 255) / 256);
+        grid_reduce_ = std::max(1, std::min((int)clist.size(), 256));
         drop_graphs();
     }
     lists_dirty_ = false;
@@ -824,7 +834,8 @@ int Engine::enqueue_step(int ph) {
         launch_tiled(fp64_, tx_, ty_, kc);
         rc = timed("diffuse_half2", rc);
     }
-    launch_ghost(fp64_, true, dim3(grid_ghosts_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g2, v_g2);
+    launch_ghost(fp64_, true, dim3(grid_ghosts_ + grid_reduce_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g2,
+                 v_g2, grid_ghosts_, d_case_list_, d_counts_ + 3);
     rc = timed("ghost2", rc);
     if (rc) return rc;
     if (timing_) {

@@ -151,10 +151,11 @@ private:
     WorkItem* d_tiles_all_ = nullptr;
     WorkItem* d_tiles_visc_ = nullptr;
     WorkItem* d_ghosts_ = nullptr;
-    int* d_counts_ = nullptr;        // [0] all tiles, [1] viscous tiles, [2] ghosts
-    size_t cap_tiles_ = 0, cap_visc_ = 0, cap_ghosts_ = 0;
+    WorkItem* d_case_list_ = nullptr;   // active cases, for the per-tile maxima reduction
+    int* d_counts_ = nullptr;        // [0] all tiles, [1] viscous tiles, [2] ghosts, [3] active cases
+    size_t cap_tiles_ = 0, cap_visc_ = 0, cap_ghosts_ = 0, cap_case_list_ = 0;
     int n_tiles_ = 0, n_visc_ = 0, n_ghosts_ = 0;
-    int grid_tiles_ = 0, grid_visc_ = 0, grid_ghosts_ = 0;
+    int grid_tiles_ = 0, grid_visc_ = 0, grid_ghosts_ = 0, grid_reduce_ = 1;
     bool any_visc_ = false;
     bool persistent_ = false;
     int persistent_grid_ = 2048;

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
     // five flux components + the interface velocity of the -p div(v) source
     // (mass flux over the mean clamped density of the two cells, solver.cu:1787-1794)
     __shared__ T s_f[6][FMAX];
-    __shared__ float s_red[NT / 64];
+    __shared__ float s_red[1][NT / 64];
 
     const int n_items = *count;
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
@@ -433,7 +433,10 @@ __global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ ca
 #pragma unroll
             for (int f = 0; f < 5; f++) ((T*)cd.b.U[r.out][f])[g] = out[f];
         }
-        if (MODE == 1 && !visc) block_max_atomic<NT>(vmax_local, s_red, &cd.b.slots[r.slot_w * 2]);
+        if (MODE == 1 && !visc) {
+            const float vm[1] = {vmax_local};
+            block_max_store<NT, 1>(vm, s_red, cd.b.tile_max, cd.tiles_x * cd.tiles_y, it.index);
+        }
         __syncthreads();
     }
 }
@@ -514,12 +517,69 @@ __global__ __launch_bounds__(NT) void k_ghost_prep(const CaseDesc<T>* __restrict
 // ghost cell is read at its pre-BC value (set pre0, or the SSP blend of pre0
 // and pre1); any other neighbour at its final value in set out.
 // FINAL: also the CFL maxima of the filled cells.
+// Wave-level maximum into a case's CFL slot counter: when every lane of the
+// wave works on the same case, one atomic per wave; otherwise one per lane.
+// Called by whole waves.
+__device__ __forceinline__ void wave_slot_max(unsigned* target, bool active, int case_id, float v) {
+    const int c0 = __shfl(case_id, 0, 64);
+    const bool uniform = __all(!active || case_id == c0);
+    if (uniform) {
+        float m = active ? v : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(target, __float_as_uint(m));
+    } else if (active && v > 0.0f) {
+        atomicMax(target, __float_as_uint(v));
+    }
+}
+
+// Workgroups [gblocks, gridDim.x) of the final ghost launch: reduce each
+// listed case's per-tile maxima (written by the step's last tiled kernel with
+// plain stores) into its CFL slot, one atomic per case and quantity.
+template <class T>
+__device__ void reduce_tile_maxima(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ case_list,
+                                   int ncases, int rblock, int nrblocks, Roles ri, Roles rv) {
+    __shared__ float s_red[2][NT / 64];
+    for (int ci = rblock; ci < ncases; ci += nrblocks) {
+        const CaseDesc<T>& cd = cases[case_list[ci].case_id];
+        const Roles r = cd.c.visc ? rv : ri;
+        const int nt = cd.tiles_x * cd.tiles_y;
+        const float* tm = cd.b.tile_max;
+        float m0 = 0.0f, m1 = 0.0f;
+        for (int q = threadIdx.x; q < nt; q += NT) {
+            m0 = fmaxf(m0, tm[q]);
+            m1 = fmaxf(m1, tm[nt + q]);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            m0 = fmaxf(m0, __shfl_xor(m0, off, 64));
+            m1 = fmaxf(m1, __shfl_xor(m1, off, 64));
+        }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) { s_red[0][wave] = m0; s_red[1][wave] = m1; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float a = s_red[0][0], c = s_red[1][0];
+            for (int w = 1; w < NT / 64; w++) { a = fmaxf(a, s_red[0][w]); c = fmaxf(c, s_red[1][w]); }
+            if (a > 0.0f) atomicMax(&cd.b.slots[r.slot_w * 2], __float_as_uint(a));
+            if (c > 0.0f) atomicMax(&cd.b.slots[r.slot_w * 2 + 1], __float_as_uint(c));
+        }
+        __syncthreads();
+    }
+}
+
 template <class T, bool FINAL>
 __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ items,
-                                              const int* __restrict__ count, Roles ri, Roles rv) {
+                                              const int* __restrict__ count, Roles ri, Roles rv, int gblocks,
+                                              const WorkItem* __restrict__ case_list, const int* __restrict__ ncases) {
+    if (FINAL && (int)blockIdx.x >= gblocks) {
+        reduce_tile_maxima(cases, case_list, *ncases, (int)blockIdx.x - gblocks, (int)gridDim.x - gblocks, ri, rv);
+        return;
+    }
     const int n_items = *count;
-    for (int w = blockIdx.x * NT + threadIdx.x; w < n_items; w += gridDim.x * NT) {
-        const WorkItem it = items[w];
+    // uniform trip count so that whole waves reach the slot reduction together
+    for (int base = blockIdx.x * NT; base < n_items; base += gblocks * NT) {
+        const int w = base + threadIdx.x;
+        const bool active = w < n_items;
+        const WorkItem it = items[active ? w : base];
         const CaseDesc<T>& cd = cases[it.case_id];
         const CaseConsts<T>& k = cd.c;
         const Roles r = k.visc ? rv : ri;
@@ -591,18 +651,21 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
             T keg = T(0.5) * rg * (ug * ug + vg * vg);
             o[0] = rg; o[1] = rg * ug; o[2] = rg * vg; o[3] = eg + keg; o[4] = eg;
         }
-        for (int f = 0; f < 5; f++) ((T*)b.U[r.out][f])[idx] = o[f];
+        if (active)
+            for (int f = 0; f < 5; f++) ((T*)b.U[r.out][f])[idx] = o[f];
         if (FINAL) {
-            float s = (float)wave_speed(k, o);
-            slot_max(&b.slots[r.slot_w * 2], s);
-            if (k.visc) {
+            const float s = active ? (float)wave_speed(k, o) : 0.0f;
+            float nu = 0.0f;
+            if (active && k.visc) {
                 T pv[5];
                 for (int f = 0; f < 5; f++) pv[f] = pre_val<T>(b, r, idx, f);
                 T u_, v_, p_, t_;
                 primitives(k, pv[0], pv[1], pv[2], pv[3], pv[4], u_, v_, p_, t_);
-                float nu = (float)(sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f)));
-                slot_max(&b.slots[r.slot_w * 2 + 1], nu);
+                nu = (float)(sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f)));
             }
+            // every lane takes part; a lane of another case keeps its own counter
+            wave_slot_max(&b.slots[r.slot_w * 2], active, it.case_id, s);
+            wave_slot_max(&b.slots[r.slot_w * 2 + 1], active && k.visc, it.case_id, nu);
         }
     }
 }
@@ -734,8 +797,8 @@ __global__ __launch_bounds__(NT) void k_diffuse(const CaseDesc<T>* __restrict__ 
             for (int f = 0; f < 5; f++) outp[f][g] = v[f];
         }
         if (FINAL) {
-            block_max_atomic<NT>(smax, s_red[0], &cd.b.slots[r.slot_w * 2]);
-            block_max_atomic<NT>(numax, s_red[1], &cd.b.slots[r.slot_w * 2 + 1]);
+            const float vm[2] = {smax, numax};
+            block_max_store<NT, 2>(vm, s_red, cd.b.tile_max, cd.tiles_x * cd.tiles_y, it.index);
         }
         __syncthreads();
     }
@@ -996,13 +1059,14 @@ bool tile_supported(int tx, int ty, bool fp64) {
 }
 
 void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
-                  const int* n, Roles ri, Roles rv) {
+                  const int* n, Roles ri, Roles rv, int gblocks, const WorkItem* case_list, const int* ncases) {
+    if (!final_) gblocks = (int)grid.x;
     if (fp64) {
-        if (final_) k_ghost<double, true><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv);
-        else k_ghost<double, false><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv);
+        if (final_) k_ghost<double, true><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv, gblocks, case_list, ncases);
+        else k_ghost<double, false><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv, gblocks, case_list, ncases);
     } else {
-        if (final_) k_ghost<float, true><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv);
-        else k_ghost<float, false><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv);
+        if (final_) k_ghost<float, true><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv, gblocks, case_list, ncases);
+        else k_ghost<float, false><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv, gblocks, case_list, ncases);
     }
 }
 

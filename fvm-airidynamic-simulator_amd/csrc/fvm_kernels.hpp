@@ -21,8 +21,12 @@ struct KernelCall {
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc);
 bool tile_supported(int tx, int ty, bool fp64);
+// Ghost fill over a list of ghost cells.  final_: the step's last launch,
+// which also takes the ghost cells' CFL maxima and, in workgroups
+// [gblocks, grid.x), reduces the per-tile maxima of the cases in case_list.
 void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
-                  const int* n, Roles ri, Roles rv);
+                  const int* n, Roles ri, Roles rv, int gblocks = 0, const WorkItem* case_list = nullptr,
+                  const int* ncases = nullptr);
 void launch_ghost_prep(bool fp64, hipStream_t s, const void* c, const WorkItem* items, int n);
 void launch_mark(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int fixup, int fix_set);
 void launch_init(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set_a, int set_b);
