@@ -225,8 +225,22 @@ __global__ __launch_bounds__(NT) void k_bc_inplace(const CaseDesc<T>* __restrict
 // KAHAN: the final blend is written as U^n + delta with a carried compensation.
 // ri / rv: roles for inviscid / viscous cases.
 // ---------------------------------------------------------------------------
+// Waves per SIMD asked of the register allocator for k_stage: six for FP32
+// (at most 80 VGPRs; measured faster than the five an unconstrained allocation
+// allows) unless the tile's LDS already admits fewer workgroups per CU (160
+// KiB of LDS, one wave per SIMD per 256-thread workgroup); FP64 keeps its
+// registers.
+template <class T, int TX, int TY>
+constexpr int stage_waves() {
+    constexpr int SX = TX + 4, SY = TY + 4, FX = (TX + 1) * TY, FY = TX * (TY + 1);
+    constexpr int FMAX = FX > FY ? FX : FY;
+    constexpr int lds = 4 * SX * SY * (int)sizeof(T) + SX * SY + 6 * FMAX * (int)sizeof(T) + 64;
+    constexpr int by_lds = 163840 / lds;
+    return sizeof(T) != 4 ? 1 : (by_lds < 6 ? by_lds : 6);
+}
+
 template <class T, int TX, int TY, int MODE, bool KAHAN>
-__global__ __launch_bounds__(NT) void k_stage(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
+__global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
                                               const int* __restrict__ count, Roles ri, Roles rv) {
     constexpr int SX = TX + 4, SY = TY + 4, CPT = (TX * TY + NT - 1) / NT;
     constexpr int FX = (TX + 1) * TY, FY = TX * (TY + 1);
