@@ -362,6 +362,25 @@ int fvm_forces(fvm_solver* s, double out3[3]) {
     return s->eng->forces(s->case_id, out3);
 }
 
+int fvm_save_snapshot(fvm_solver* s, const char* path) {
+    CHECK_SOLVER(s);
+    return s->eng->save_case(s->case_id, path);
+}
+
+int fvm_load_snapshot(fvm_solver* s, const char* path, fvm_params* params_out) {
+    if (!s) return set_error(FVM_ERR_INVALID_ARGUMENT, "null solver handle");
+    fvm_snapshot_header h;
+    fvm_params p;
+    int rc = fvm::read_snapshot_header(path, &h, &p);
+    if (rc) return rc;
+    rc = fvm_initialize(s, &p);
+    if (rc) return rc;
+    rc = s->eng->load_case(s->case_id, path);
+    if (rc) return rc;
+    if (params_out) *params_out = p;
+    return FVM_OK;
+}
+
 int fvm_get_time(fvm_solver* s, double* t, int64_t* steps) {
     CHECK_SOLVER(s);
     long long n = 0;
@@ -553,6 +572,29 @@ int fvm_batch_get_cell_types(fvm_batch* b, int32_t case_id, uint8_t* host) {
 int fvm_batch_forces(fvm_batch* b, int32_t case_id, double out3[3]) {
     if (!b || !out3) return set_error(FVM_ERR_INVALID_ARGUMENT, "null argument");
     return b->eng->forces(case_id, out3);
+}
+
+int fvm_batch_save_case(fvm_batch* b, int32_t case_id, const char* path) {
+    if (!b) return set_error(FVM_ERR_INVALID_ARGUMENT, "null batch");
+    return b->eng->save_case(case_id, path);
+}
+
+int fvm_batch_load_case(fvm_batch* b, const char* path, int32_t steps_total, int32_t* case_id) {
+    if (!b || !case_id) return set_error(FVM_ERR_INVALID_ARGUMENT, "null argument");
+    fvm_snapshot_header h;
+    fvm_params p;
+    int rc = fvm::read_snapshot_header(path, &h, &p);
+    if (rc) return rc;
+    int32_t id = -1;
+    rc = fvm_batch_add_case(b, &p, steps_total, &id);
+    if (rc) return rc;
+    rc = b->eng->load_case(id, path);
+    if (rc) {
+        (void)b->eng->remove_case(id);
+        return rc;
+    }
+    *case_id = id;
+    return FVM_OK;
 }
 
 int fvm_batch_active_cases(fvm_batch* b, int32_t* n) {

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdio>
 #include <cstring>
 
 #pragma clang fp contract(off)
@@ -1086,6 +1087,112 @@ int Engine::forces(int id, double out[3]) {
     double r[4];
     FVM_HIP(hipMemcpy(r, d_cons_, 4 * sizeof(double), hipMemcpyDeviceToHost));
     out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
+    return FVM_OK;
+}
+
+// ------------------------------------------------------------- snapshots
+namespace {
+struct File {
+    FILE* f = nullptr;
+    explicit File(FILE* p) : f(p) {}
+    ~File() { if (f) std::fclose(f); }
+};
+}  // namespace
+
+static_assert(sizeof(fvm_snapshot_header) == 256, "snapshot header is 256 bytes");
+
+int read_snapshot_header(const char* path, fvm_snapshot_header* h, fvm_params* p) {
+    if (!path) return set_error(FVM_ERR_INVALID_ARGUMENT, "null path");
+    File in(std::fopen(path, "rb"));
+    if (!in.f) return set_error(FVM_ERR_INVALID_ARGUMENT, std::string("cannot open snapshot ") + path);
+    if (std::fread(h, sizeof(*h), 1, in.f) != 1) return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot header truncated");
+    if (std::memcmp(h->magic, FVM_SNAPSHOT_MAGIC, 8) != 0 || h->version != FVM_SNAPSHOT_VERSION ||
+        h->header_bytes != sizeof(fvm_snapshot_header))
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "not an fvm snapshot of this version");
+    if (h->abi_version != FVM_ABI_VERSION || h->params_bytes != sizeof(fvm_params))
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot written by another ABI version");
+    if (h->nx <= 0 || h->ny <= 0 || (h->elem_bytes != 4 && h->elem_bytes != 8))
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot header has bad sizes");
+    if (p && std::fread(p, sizeof(*p), 1, in.f) != 1) return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot params truncated");
+    if (p && (p->nx != h->nx || p->ny != h->ny)) return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot sizes disagree");
+    return FVM_OK;
+}
+
+int Engine::save_case(int id, const char* path) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    if (!path) return set_error(FVM_ERR_INVALID_ARGUMENT, "null path");
+    FVM_HIP(hipStreamSynchronize(stream_));
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    char* base = (char*)hc->mem;
+    fvm_snapshot_header h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, FVM_SNAPSHOT_MAGIC, 8);
+    h.version = FVM_SNAPSHOT_VERSION;
+    h.header_bytes = sizeof(h);
+    h.abi_version = FVM_ABI_VERSION;
+    h.params_bytes = sizeof(fvm_params);
+    h.nx = hc->p.nx;
+    h.ny = hc->p.ny;
+    h.elem_bytes = fp64_ ? 8 : 4;
+    h.kahan = kahan_ ? 1 : 0;
+    long long steps = 0;
+    FVM_HIP(hipMemcpy(&h.t, base + L.t_off, 8, hipMemcpyDeviceToHost));
+    FVM_HIP(hipMemcpy(&steps, base + L.steps_off, 8, hipMemcpyDeviceToHost));
+    h.steps = steps;
+    unsigned bits[2];
+    FVM_HIP(hipMemcpy(bits, base + L.slots_off + phase_ * 8, 8, hipMemcpyDeviceToHost));
+    h.max_speed_bits = bits[0];
+    h.max_nu_bits = bits[1];
+    File out(std::fopen(path, "wb"));
+    if (!out.f) return set_error(FVM_ERR_INVALID_ARGUMENT, std::string("cannot create snapshot ") + path);
+    bool ok = std::fwrite(&h, sizeof(h), 1, out.f) == 1 && std::fwrite(&hc->p, sizeof(fvm_params), 1, out.f) == 1;
+    const int cur = current_set(*hc);
+    std::vector<char> buf(L.n * h.elem_bytes);
+    for (int f = 0; ok && f < 5; f++) {
+        FVM_HIP(hipMemcpy(buf.data(), base + L.sets_off + L.plane * (cur * 5 + f), buf.size(), hipMemcpyDeviceToHost));
+        ok = std::fwrite(buf.data(), 1, buf.size(), out.f) == buf.size();
+    }
+    for (int f = 0; ok && kahan_ && f < 5; f++) {
+        FVM_HIP(hipMemcpy(buf.data(), base + L.k_off + round256(L.n * 4) * f, L.n * 4, hipMemcpyDeviceToHost));
+        ok = std::fwrite(buf.data(), 1, L.n * 4, out.f) == L.n * 4;
+    }
+    if (!ok || std::fflush(out.f) != 0) return set_error(FVM_ERR_INVALID_ARGUMENT, std::string("write failed: ") + path);
+    return FVM_OK;
+}
+
+int Engine::load_case(int id, const char* path) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    fvm_snapshot_header h;
+    fvm_params p;
+    int rc = read_snapshot_header(path, &h, &p);
+    if (rc) return rc;
+    if (h.nx != hc->p.nx || h.ny != hc->p.ny || h.elem_bytes != (fp64_ ? 8u : 4u) || (h.kahan != 0) != kahan_)
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot does not match the case (size, precision or carries)");
+    File in(std::fopen(path, "rb"));
+    if (!in.f || std::fseek(in.f, (long)(sizeof(h) + sizeof(fvm_params)), SEEK_SET) != 0)
+        return set_error(FVM_ERR_INVALID_ARGUMENT, std::string("cannot read snapshot ") + path);
+    FVM_HIP(hipStreamSynchronize(stream_));
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    char* base = (char*)hc->mem;
+    const int cur = current_set(*hc);
+    std::vector<char> buf(L.n * h.elem_bytes);
+    for (int f = 0; f < 5; f++) {
+        if (std::fread(buf.data(), 1, buf.size(), in.f) != buf.size())
+            return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot state truncated");
+        FVM_HIP(hipMemcpy(base + L.sets_off + L.plane * (cur * 5 + f), buf.data(), buf.size(), hipMemcpyHostToDevice));
+    }
+    for (int f = 0; kahan_ && f < 5; f++) {
+        if (std::fread(buf.data(), 1, L.n * 4, in.f) != L.n * 4)
+            return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot carries truncated");
+        FVM_HIP(hipMemcpy(base + L.k_off + round256(L.n * 4) * f, buf.data(), L.n * 4, hipMemcpyHostToDevice));
+    }
+    long long steps = h.steps;
+    unsigned bits[2] = {h.max_speed_bits, h.max_nu_bits};
+    FVM_HIP(hipMemcpy(base + L.t_off, &h.t, 8, hipMemcpyHostToDevice));
+    FVM_HIP(hipMemcpy(base + L.steps_off, &steps, 8, hipMemcpyHostToDevice));
+    FVM_HIP(hipMemcpy(base + L.slots_off + phase_ * 8, bits, 8, hipMemcpyHostToDevice));
     return FVM_OK;
 }
 

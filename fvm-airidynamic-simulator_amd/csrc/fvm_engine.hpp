@@ -17,6 +17,8 @@
 namespace fvm {
 
 int set_error(int code, const std::string& msg);
+// Validates a snapshot file's header and reads its parameters (p may be null).
+int read_snapshot_header(const char* path, fvm_snapshot_header* h, fvm_params* p);
 int hip_fail(hipError_t e, const char* what);
 
 #define FVM_HIP(call)                                  \
@@ -107,6 +109,8 @@ public:
     int stat(int id, int mode, float* out);
     int conservation(int id, double out[4]);
     int forces(int id, double out[3]);   // Fx, Fy, number of wall faces
+    int save_case(int id, const char* path);   // snapshot (fvm_snapshot_header layout)
+    int load_case(int id, const char* path);   // case must have the snapshot's size and precision
 
     int active_count() const;
     int case_slots() const { return (int)cases_.size(); }

@@ -7,4 +7,4 @@ the synthetic workloads.
 from . import params  # noqa: F401
 from .lib import FvmError, LIB_PATH, lib  # noqa: F401
 from .solver import CFDSolver  # noqa: F401
-from . import batch, diagnostics, serve  # noqa: F401
+from . import batch, diagnostics, serve, snapshot  # noqa: F401

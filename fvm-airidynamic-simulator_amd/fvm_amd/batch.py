@@ -68,6 +68,19 @@ class BatchEngine:
               "fvm_batch_get_state")
         return out
 
+    def save(self, cid: int, path: str):
+        check(lib().fvm_batch_save_case(self._h, cid, str(path).encode()), "fvm_batch_save_case")
+
+    def load(self, path: str, steps_total: int = 0) -> int:
+        """Admit a case from a snapshot file; it runs steps_total more steps."""
+        from .snapshot import read
+        cid = ctypes.c_int32()
+        check(lib().fvm_batch_load_case(self._h, str(path).encode(), steps_total, ctypes.byref(cid)),
+              "fvm_batch_load_case")
+        h = read(path).header
+        self._shape[cid.value] = (h.ny, h.nx)
+        return cid.value
+
     def forces(self, cid: int) -> np.ndarray:
         out = np.zeros(3, dtype=np.float64)
         check(lib().fvm_batch_forces(self._h, cid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),

@@ -61,6 +61,8 @@ SIGNATURES = [
     ("fvm_conservation_sums", _i, [_vp, _dp]),
     ("fvm_forces", _i, [_vp, _dp]),
     ("fvm_get_time", _i, [_vp, _dp, _i64p]),
+    ("fvm_save_snapshot", _i, [_vp, ctypes.c_char_p]),
+    ("fvm_load_snapshot", _i, [_vp, ctypes.c_char_p, _P]),
     ("fvm_max_temperature", _i, [_vp, _fp]),
     ("fvm_max_mach", _i, [_vp, _fp]),
     ("fvm_memory_usage", _i, [_vp, _szp]),
@@ -80,6 +82,8 @@ SIGNATURES = [
     ("fvm_batch_get_state", _i, [_vp, ctypes.c_int32, _fp]),
     ("fvm_batch_get_cell_types", _i, [_vp, ctypes.c_int32, _u8p]),
     ("fvm_batch_forces", _i, [_vp, ctypes.c_int32, _dp]),
+    ("fvm_batch_save_case", _i, [_vp, ctypes.c_int32, ctypes.c_char_p]),
+    ("fvm_batch_load_case", _i, [_vp, ctypes.c_char_p, ctypes.c_int32, _i32p]),
     ("fvm_batch_active_cases", _i, [_vp, _i32p]),
     ("fvm_batch_bytes_in_use", _i, [_vp, _szp]),
     ("fvm_batch_synchronize", _i, [_vp]),

@@ -152,6 +152,16 @@ class CFDSolver:
         check(lib().fvm_forces(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "fvm_forces")
         return out
 
+    def save_snapshot(self, path: str):
+        check(lib().fvm_save_snapshot(self._h, str(path).encode()), "fvm_save_snapshot")
+
+    def load_snapshot(self, path: str) -> P.FvmParams:
+        """Initialise from a snapshot file; returns its parameters."""
+        p = P.FvmParams()
+        check(lib().fvm_load_snapshot(self._h, str(path).encode(), ctypes.byref(p)), "fvm_load_snapshot")
+        self._nx, self._ny = p.nx, p.ny
+        return p
+
     def time(self):
         t = ctypes.c_double()
         n = ctypes.c_int64()

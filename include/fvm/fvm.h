@@ -193,6 +193,43 @@ int fvm_forces(fvm_solver* s, double out3[3]);
 /* Device-side simulated time (FP64 sum of the dt of every step since reset)
  * and step count. */
 int fvm_get_time(fvm_solver* s, double* t, int64_t* steps);
+
+/*
+ * State snapshot (no reference counterpart: the reference keeps its state on
+ * the device only).  File layout, little-endian:
+ *   fvm_snapshot_header (256 bytes)
+ *   fvm_params          (header.params_bytes bytes)
+ *   5 planes rho, rho_u, rho_v, E, rho_e of nx*ny values, row-major
+ *     idx = j*nx + i, header.elem_bytes each (4: FP32 state, 8: FP64 state)
+ *   5 planes of FP32 blend compensation carries when header.kahan != 0
+ * The header also keeps the device time, the step count and the CFL maxima
+ * the next step's dt is computed from, so a loaded snapshot continues exactly
+ * like the run that wrote it.
+ */
+#define FVM_SNAPSHOT_MAGIC "FVMSNAP"
+#define FVM_SNAPSHOT_VERSION 1
+typedef struct fvm_snapshot_header {
+    char magic[8];              /* "FVMSNAP" + NUL */
+    uint32_t version;           /* FVM_SNAPSHOT_VERSION */
+    uint32_t header_bytes;      /* sizeof(fvm_snapshot_header) = 256 */
+    uint32_t abi_version;       /* FVM_ABI_VERSION of the writer */
+    uint32_t params_bytes;      /* sizeof(fvm_params) */
+    int32_t nx, ny;
+    uint32_t elem_bytes;        /* 4 or 8 */
+    uint32_t kahan;             /* 1: carry planes follow the state */
+    double t;                   /* device simulated time */
+    int64_t steps;              /* device step count */
+    uint32_t max_speed_bits;    /* CFL maxima of the state (float bit patterns) */
+    uint32_t max_nu_bits;
+    uint8_t reserved[192];
+} fvm_snapshot_header;
+
+/* Writes the current state of the solver to `path`. */
+int fvm_save_snapshot(fvm_solver* s, const char* path);
+/* Reads `path`: initialises the solver with the stored parameters (as
+ * fvm_initialize), then restores state, time, step count, CFL maxima and
+ * carries.  params_out (may be NULL) receives the stored parameters. */
+int fvm_load_snapshot(fvm_solver* s, const char* path, fvm_params* params_out);
 /* getMaxTemperature / getMaxMach (solver.cu:3417-3426). */
 int fvm_max_temperature(fvm_solver* s, float* out);
 int fvm_max_mach(fvm_solver* s, float* out);
@@ -243,6 +280,11 @@ int fvm_batch_get_state(fvm_batch* b, int32_t case_id, float* host5);
 int fvm_batch_get_cell_types(fvm_batch* b, int32_t case_id, uint8_t* host);
 /* fvm_forces for one case of the batch. */
 int fvm_batch_forces(fvm_batch* b, int32_t case_id, double out3[3]);
+/* Snapshot of one case of the batch (same file format as fvm_save_snapshot). */
+int fvm_batch_save_case(fvm_batch* b, int32_t case_id, const char* path);
+/* Admits a case from a snapshot (fvm_batch_add_case with the stored
+ * parameters, then the stored state); it runs steps_total more steps. */
+int fvm_batch_load_case(fvm_batch* b, const char* path, int32_t steps_total, int32_t* case_id);
 int fvm_batch_active_cases(fvm_batch* b, int32_t* n);
 int fvm_batch_bytes_in_use(fvm_batch* b, size_t* bytes);
 int fvm_batch_synchronize(fvm_batch* b);

@@ -992,6 +992,24 @@ struct Oracle {
         max_speed = ms; max_nu = mn;
     }
 
+    /* updateObstacleGeometry, solver.cu:3171-3196: re-mark with the new
+     * geometry; a cell that turns from solid/ghost into fluid has its state
+     * zeroed (updateSDFWithFixupKernel, 607-664); then the boundary conditions
+     * are applied again.  The rest of the state is kept. */
+    void update_obstacle(const fvm_params& p) {
+        P = p;
+        std::vector<uint8_t> old = ct;
+        mark(p);
+        for (size_t idx = 0; idx < N(); idx++) {
+            bool was = old[idx] == FVM_CELL_SOLID || old[idx] == FVM_CELL_GHOST;
+            if (was && ct[idx] == FVM_CELL_FLUID)
+                for (int c = 0; c < 5; c++) U.f[c][idx] = (T)0;
+        }
+        apply_bc(U);
+        compute_prims(U);
+        diffuse(U, (T)0, false);   // writes mu only (dt = 0)
+    }
+
     /* Pressure force on the obstacle (per unit depth) implied by the wall
      * fluxes of computeFluxesKernel (solver.cu:1589-1598 x faces, 1673-1682
      * y faces): a face with a solid/ghost cell on either side carries the
@@ -1178,6 +1196,11 @@ void oracle_conservation_sums(fvm_oracle* o, double out[4]) {
         for (size_t i = 0; i < n; i++) s += o->fp64 ? o->d->U.f[c][i] : (double)o->f->U.f[c][i];
         out[c] = s;
     }
+}
+
+void oracle_update_obstacle(fvm_oracle* o, const fvm_params* p) {
+    if (o->fp64) o->d->update_obstacle(*p);
+    else o->f->update_obstacle(*p);
 }
 
 void oracle_get_time(fvm_oracle* o, double* t, long long* n) {

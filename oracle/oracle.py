@@ -56,6 +56,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_stable_dt.restype = ctypes.c_double
         L.oracle_conservation_sums.argtypes = [ctypes.c_void_p, dp]
         L.oracle_forces.argtypes = [ctypes.c_void_p, dp]
+        L.oracle_update_obstacle.argtypes = [ctypes.c_void_p, P]
         L.oracle_get_time.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_set_threads.restype = ctypes.c_int
@@ -152,6 +153,11 @@ class Oracle:
         out = np.zeros(4, dtype=np.float64)
         lib().oracle_conservation_sums(self._h, _dptr(out))
         return out
+
+    def update_obstacle(self, params: FvmParams):
+        """updateObstacleGeometry: new geometry, exposed cells zeroed, BC re-applied."""
+        self.params = params.copy()
+        lib().oracle_update_obstacle(self._h, ctypes.byref(self.params))
 
     def time(self):
         """(simulated time, steps) since reset."""

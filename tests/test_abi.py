@@ -7,6 +7,7 @@ import re
 import subprocess
 import tempfile
 
+import numpy as np
 import pytest
 
 import fvm_amd
@@ -83,3 +84,50 @@ def test_create_without_gpu_fails_cleanly():
     rc = fvm_amd.lib().fvm_create(0, ctypes.byref(h))
     assert rc != 0
     assert fvm_amd.lib().fvm_last_error()
+
+
+def test_snapshot_header_layout_matches_c():
+    from fvm_amd.snapshot import SnapshotHeader as H
+    fields = ["version", "header_bytes", "abi_version", "params_bytes", "nx", "ny", "elem_bytes", "kahan", "t",
+              "steps", "max_speed_bits", "max_nu_bits", "reserved"]
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"fvm/fvm.h\"\nint main(void) {\n"
+    src += '  printf("%zu", sizeof(fvm_snapshot_header));\n'
+    for f in fields:
+        src += f'  printf(" %zu", offsetof(fvm_snapshot_header, {f}));\n'
+    src += '  printf(" %d %s\\n", FVM_SNAPSHOT_VERSION, FVM_SNAPSHOT_MAGIC);\n  return 0;\n}\n'
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.check_call(["gcc", "-I", os.path.join(REPO, "include"), c, "-o", exe])
+        out = subprocess.check_output([exe]).split()
+    vals = [int(x) for x in out[:-1]]
+    assert vals[0] == ctypes.sizeof(H) == 256
+    assert vals[1:-1] == [getattr(H, f).offset for f in fields]
+    from fvm_amd import snapshot
+    assert vals[-1] == snapshot.VERSION and out[-1] + b"\x00" == snapshot.MAGIC
+
+
+def test_snapshot_reader_on_a_written_file(tmp_path):
+    """The reader parses a file laid out as the header documents and rejects
+    a truncated one."""
+    from fvm_amd import snapshot
+    p = P.euler_bow_shock(nx=12, ny=5)
+    h = snapshot.SnapshotHeader()
+    h.magic = snapshot.MAGIC
+    h.version, h.header_bytes, h.abi_version = snapshot.VERSION, 256, 1
+    h.params_bytes = ctypes.sizeof(P.FvmParams)
+    h.nx, h.ny, h.elem_bytes, h.kahan = 12, 5, 4, 1
+    h.t, h.steps = 12.5, 42
+    h.max_speed_bits = int(np.array([4.0], np.float32).view(np.uint32)[0])
+    state = np.arange(5 * 60, dtype=np.float32).reshape(5, 5, 12)
+    carries = -state * 1e-7
+    f = tmp_path / "s.fvmsnap"
+    f.write_bytes(bytes(h) + bytes(p) + state.tobytes() + carries.astype(np.float32).tobytes())
+    s = snapshot.read(str(f))
+    assert (s.header.steps, s.header.t, s.max_speed) == (42, 12.5, 4.0)
+    assert s.params.nx == 12 and s.params.mach == p.mach
+    assert np.array_equal(s.state, state) and np.array_equal(s.carries, carries.astype(np.float32))
+    f.write_bytes(f.read_bytes()[:-4])
+    with pytest.raises(ValueError):
+        snapshot.read(str(f))

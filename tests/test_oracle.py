@@ -329,3 +329,28 @@ def test_forces_zero_at_uniform_pressure():
     f = o.forces()
     assert abs(f[0]) < 1e-12 and abs(f[1]) < 1e-12
     o.close()
+
+
+def test_obstacle_update_zeroes_exposed_cells():
+    """updateSDFWithFixupKernel (solver.cu:607-664): cells that turn from
+    solid/ghost into fluid get a zero state; interior fluid cells are not
+    touched by the boundary conditions, so exactly those cells read zero after
+    the update, and the new mask equals a fresh marking of the new geometry."""
+    p = P.euler_bow_shock(nx=128, ny=64, radius=12.0, center=(40.0, 32.0))
+    p.dt = 0.1
+    o = O.Oracle(p, fp64=True)
+    o.step(5, p.copy())
+    old = o.cell_types()
+    p2 = p.copy()
+    p2.obstacle_x += 6.0
+    o.update_obstacle(p2)
+    new = o.cell_types()
+    fresh = O.Oracle(p2, fp64=True)
+    assert np.array_equal(new, fresh.cell_types())
+    exposed = np.isin(old, (P.CELL_SOLID, P.CELL_GHOST)) & (new == P.CELL_FLUID)
+    assert exposed.sum() > 0
+    s = o.state()
+    zero = np.all(s == 0.0, axis=0)
+    assert np.array_equal(zero, exposed)
+    o.close()
+    fresh.close()
