@@ -63,6 +63,8 @@ struct CaseBufs {
     uint8_t* ct;
     float* sdf;
     void* ghosts;         // GhostInfo<T>[n_ghosts]
+    int* gmap;            // ghost-table index of each cell of the ghost bounding box, -1 elsewhere
+    int gbox[4];          // bounding box of the ghost cells: i0, j0, width, height
     unsigned* slots;      // [2 slots][2]: bits of max wave speed, max mu/rho
     float* tile_max;      // [2][tiles]: per-tile max wave speed, max mu/rho of the step's final kernel
     double* t;            // simulated time

@@ -341,6 +341,8 @@ int Engine::upload_desc(HostCase& hc) {
         for (int f = 0; f < 5; f++) b.K[f] = (float*)(base + L.k_off + round256(L.n * 4) * f);
     b.ct = (uint8_t*)(base + L.ct_off);
     b.ghosts = hc.ghost_mem;
+    b.gmap = hc.ghost_mem ? (int*)((char*)hc.ghost_mem + hc.gmap_off) : nullptr;
+    for (int q = 0; q < 4; q++) b.gbox[q] = hc.gbox[q];
     b.sdf = (float*)(base + L.sdf_off);
     b.slots = (unsigned*)(base + L.slots_off);
     b.t = (double*)(base + L.t_off);
@@ -535,11 +537,22 @@ int Engine::mark_and_lists(HostCase& hc, int fixup) {
     FVM_HIP(hipMemcpyAsync(ct.data(), (char*)hc.mem + L.ct_off, n, hipMemcpyDeviceToHost, stream_));
     FVM_HIP(hipStreamSynchronize(stream_));
     hc.ghosts.clear();
+    int bi0 = hc.p.nx, bj0 = hc.p.ny, bi1 = -1, bj1 = -1;
     for (size_t i = 0; i < n; i++)
-        if (ct[i] == CT_GHOST) hc.ghosts.push_back((int)i);
-    // device table of ghost cells: cell index now, normal and ranked neighbours from k_ghost_prep
+        if (ct[i] == CT_GHOST) {
+            hc.ghosts.push_back((int)i);
+            const int ci = (int)(i % hc.p.nx), cj = (int)(i / hc.p.nx);
+            bi0 = std::min(bi0, ci); bi1 = std::max(bi1, ci);
+            bj0 = std::min(bj0, cj); bj1 = std::max(bj1, cj);
+        }
+    // device block: ghost table (cell index now; normal and ranked neighbours
+    // from k_ghost_prep), then the ghost-table index of every cell of the
+    // ghost cells' bounding box (-1 for the others)
     const size_t gsz = fp64_ ? sizeof(GhostInfo<double>) : sizeof(GhostInfo<float>);
-    if (hc.ghosts.size() > hc.ghost_cap) {
+    const int bw = hc.ghosts.empty() ? 0 : bi1 - bi0 + 1, bh = hc.ghosts.empty() ? 0 : bj1 - bj0 + 1;
+    const size_t table_bytes = round256(hc.ghosts.size() * gsz);
+    const size_t need = table_bytes + (size_t)bw * bh * sizeof(int);
+    if (need > hc.ghost_cap) {
         if (hc.ghost_mem) {
             FVM_HIP(hipStreamSynchronize(stream_));
             if (use_arena_) arena_.release(hc.ghost_mem);
@@ -547,23 +560,36 @@ int Engine::mark_and_lists(HostCase& hc, int fixup) {
             hc.ghost_mem = nullptr;
             hc.ghost_cap = 0;
         }
-        size_t cap = hc.ghosts.size() + hc.ghosts.size() / 4 + 16;
+        size_t cap = round256(need + need / 4 + 256);
         if (use_arena_) {
-            hc.ghost_mem = arena_.alloc(cap * gsz);
+            hc.ghost_mem = arena_.alloc(cap);
             if (!hc.ghost_mem) return set_error(FVM_ERR_CAPACITY, "device memory budget exhausted (ghost table)");
         } else {
-            FVM_HIP(hipMalloc(&hc.ghost_mem, cap * gsz));
+            FVM_HIP(hipMalloc(&hc.ghost_mem, cap));
         }
         hc.ghost_cap = cap;
-        hc.bytes = L.total + cap * gsz;
+        hc.bytes = L.total + cap;
+    }
+    hc.gbox[0] = bw ? bi0 : 0;
+    hc.gbox[1] = bh ? bj0 : 0;
+    hc.gbox[2] = bw;
+    hc.gbox[3] = bh;
+    hc.gmap_off = table_bytes;
+    if (!hc.ghosts.empty()) {
+        std::vector<char> blk(need, 0);
+        for (size_t g = 0; g < hc.ghosts.size(); g++) std::memcpy(blk.data() + g * gsz, &hc.ghosts[g], sizeof(int));
+        int* map = (int*)(blk.data() + table_bytes);
+        std::fill(map, map + (size_t)bw * bh, -1);
+        for (size_t g = 0; g < hc.ghosts.size(); g++) {
+            const int ci = hc.ghosts[g] % hc.p.nx, cj = hc.ghosts[g] / hc.p.nx;
+            map[(size_t)(cj - bj0) * bw + (ci - bi0)] = (int)g;
+        }
+        FVM_HIP(hipMemcpyAsync(hc.ghost_mem, blk.data(), blk.size(), hipMemcpyHostToDevice, stream_));
+        FVM_HIP(hipStreamSynchronize(stream_));
+    }
+    {
         int rc0 = upload_desc(hc);
         if (rc0) return rc0;
-    }
-    if (!hc.ghosts.empty()) {
-        std::vector<char> gtab(hc.ghosts.size() * gsz, 0);
-        for (size_t g = 0; g < hc.ghosts.size(); g++) std::memcpy(gtab.data() + g * gsz, &hc.ghosts[g], sizeof(int));
-        FVM_HIP(hipMemcpyAsync(hc.ghost_mem, gtab.data(), gtab.size(), hipMemcpyHostToDevice, stream_));
-        FVM_HIP(hipStreamSynchronize(stream_));
     }
 
     int cur = current_set(hc);
@@ -669,18 +695,33 @@ int Engine::rebuild_lists() {
     const bool had_visc = any_visc_;
     WorkItem* const old_ptrs[4] = {d_tiles_all_, d_tiles_visc_, d_ghosts_, d_case_list_};
     any_visc_ = false;
+    std::vector<WorkItem> heavy;
     for (HostCase* c : cases_) {
         if (!c || !c->active || c->finished) continue;
         int nt = c->tiles_x * c->tiles_y;
+        // tiles whose staged region (tile and 2-cell halo) holds a ghost cell:
+        // stage 2 computes those ghost values, a chain of dependent loads
+        std::vector<char> h((size_t)nt, 0);
+        for (int g : c->ghosts) {
+            const int ci = g % c->p.nx, cj = g / c->p.nx;
+            const int tx0 = std::max(0, (ci - 2) / tx_), tx1 = std::min(c->tiles_x - 1, (ci + 2) / tx_);
+            const int ty0 = std::max(0, (cj - 2) / ty_), ty1 = std::min(c->tiles_y - 1, (cj + 2) / ty_);
+            for (int b = ty0; b <= ty1; b++)
+                for (int a = tx0; a <= tx1; a++) h[(size_t)b * c->tiles_x + a] = 1;
+        }
         for (int t = 0; t < nt; t++) {
-            all.push_back(WorkItem{c->id, t});
+            if (h[t]) heavy.push_back(WorkItem{c->id, t});
+            else all.push_back(WorkItem{c->id, t});
             if (c->p.enable_viscosity) visc.push_back(WorkItem{c->id, t});
         }
         if (c->p.enable_viscosity) any_visc_ = true;
         for (int g = 0; g < (int)c->ghosts.size(); g++) ghosts.push_back(WorkItem{c->id, g});
         clist.push_back(WorkItem{c->id, 0});
     }
+    // heavy tiles first: dispatched before the rest and spread over the XCDs
+    // (dispatch slots go round-robin over them); then the XCD-contiguous order
     xcd_order(all, oall);
+    oall.insert(oall.begin(), heavy.begin(), heavy.end());
     xcd_order(visc, ovisc);
     int rc = upload_items(oall, &d_tiles_all_, &cap_tiles_, d_counts_, 0, stream_);
     if (!rc) rc = upload_items(ovisc, &d_tiles_visc_, &cap_visc_, d_counts_, 1, stream_);
@@ -784,12 +825,10 @@ int Engine::enqueue_step(int ph) {
     const int cur = ph ? 2 : 0, nxt = ph ? 0 : 2;
     const int sr = ph, sw = 1 - ph;
     Roles i_s1{cur, 1, -1, -1, -1, sr, sw, 1, 1};
-    Roles i_g1{-1, 1, -1, cur, -1, sr, sw, 0, 0};
     Roles i_s2{1, nxt, cur, -1, -1, sr, sw, 0, 0};
     Roles i_g2{-1, nxt, -1, cur, 1, sr, sw, 0, 0};
     Roles v_k0{0, 3, -1, -1, -1, sr, sw, 1, 1};
     Roles v_s1{3, 1, -1, -1, -1, sr, sw, 0, 0};
-    Roles v_g1{-1, 1, -1, 3, -1, sr, sw, 0, 0};
     Roles v_s2{1, 2, 3, -1, -1, sr, sw, 0, 0};
     Roles v_k3{2, 0, -1, -1, -1, sr, sw, 0, 0};
     Roles v_g2{-1, 0, -1, 2, -1, sr, sw, 0, 0};
@@ -818,8 +857,6 @@ int Engine::enqueue_step(int ph) {
     kc.rv = v_s1;
     launch_tiled(fp64_, tx_, ty_, kc);
     rc = timed("stage1", rc);
-    launch_ghost(fp64_, false, dim3(grid_ghosts_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g1, v_g1);
-    rc = timed("ghost1", rc);
     kc.op = OP_STAGE2;
     kc.ri = i_s2;
     kc.rv = v_s2;

@@ -55,8 +55,10 @@ struct HostCase {
     size_t bytes = 0;
     void* mem = nullptr;
     std::vector<int> ghosts;      // cell index of every ghost cell
-    void* ghost_mem = nullptr;    // GhostInfo<T>[capacity] on the device
-    size_t ghost_cap = 0;
+    void* ghost_mem = nullptr;    // GhostInfo<T> table, then the ghost index map of the bounding box
+    size_t ghost_cap = 0;         // bytes
+    int gbox[4] = {0, 0, 0, 0};   // ghost bounding box: i0, j0, width, height
+    size_t gmap_off = 0;          // byte offset of the index map in ghost_mem
     int tiles_x = 0, tiles_y = 0;
     int steps_total = 0;   // batch: steps to run before completion (0 = unbounded)
     long long steps_done = 0;

@@ -225,6 +225,106 @@ __global__ __launch_bounds__(NT) void k_bc_inplace(const CaseDesc<T>* __restrict
 // KAHAN: the final blend is written as U^n + delta with a carried compensation.
 // ri / rv: roles for inviscid / viscous cases.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Immersed-boundary ghost value (applyBoundaryConditionsKernel ghost branch,
+// src/solver.cu:2045-2225).  A neighbour that is itself a ghost cell is read
+// at its pre-BC value (set pre0, or the SSP blend of pre0 and pre1); any other
+// neighbour at its post-BC value in set src.  Normal and ranked neighbours come
+// from the ghost table (static per geometry).  IEEE arithmetic, no contraction,
+// as in the oracle.
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+__constant__ int c_ox[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+__constant__ int c_oy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+
+template <class T>
+__device__ __forceinline__ T pre_val(const CaseBufs& b, int pre0, int pre1, size_t n, int f) {
+    T a = ((const T*)b.U[pre0][f])[n];
+    if (pre1 < 0) return a;
+    T c = ((const T*)b.U[pre1][f])[n];
+    return T(0.5) * a + T(0.5) * c;
+}
+
+// Index into the ghost table of cell (i, j), or -1 when it is not a ghost cell.
+__device__ __forceinline__ int ghost_index(const CaseBufs& b, int i, int j) {
+    const int li = i - b.gbox[0], lj = j - b.gbox[1];
+    if (li < 0 || lj < 0 || li >= b.gbox[2] || lj >= b.gbox[3]) return -1;
+    return b.gmap[lj * b.gbox[2] + li];
+}
+
+template <class T>
+__device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, int gidx, T o[5]) {
+    const CaseConsts<T>& k = cd.c;
+    const CaseBufs& b = cd.b;
+    const GhostInfo<T> gi = ((const GhostInfo<T>*)b.ghosts)[gidx];
+    const int nx = k.nx;
+    const size_t idx = (size_t)gi.idx;
+    const uint8_t* ct = b.ct;
+    const T nxs = gi.nx, nys = gi.ny;
+    const int cnt = (int)(gi.order >> 24);
+    long long best = -1;
+    bool best_ghost = false;
+    for (int q = 0; q < cnt; q++) {
+        int m = (gi.order >> (3 * q)) & 7;
+        size_t nidx = (size_t)((long long)idx + c_oy[m] * (long long)nx + c_ox[m]);
+        bool g = ct[nidx] == CT_GHOST;
+        T rn = g ? pre_val<T>(b, pre0, pre1, nidx, 0) : ((const T*)b.U[src][0])[nidx];
+        if (rn >= k.min_rho) {
+            best = (long long)nidx;
+            best_ghost = g;
+            break;
+        }
+    }
+    if (best < 0) {
+        o[0] = k.rho_inf; o[1] = T(0); o[2] = T(0); o[3] = k.e_inf; o[4] = k.e_inf;
+        return;
+    }
+    size_t bn = (size_t)best;
+    T fv[4];
+    for (int f = 0; f < 4; f++) fv[f] = best_ghost ? pre_val<T>(b, pre0, pre1, bn, f) : ((const T*)b.U[src][f])[bn];
+    T rho_f = fv[0];
+    T u_f = fv[1] / rho_f, v_f = fv[2] / rho_f;
+    T ke_f = T(0.5) * rho_f * (u_f * u_f + v_f * v_f);
+    T p_f = k.gm1 * (fv[3] - ke_f);
+    rho_f = tmax(rho_f, k.min_rho);
+    p_f = tmax(p_f, k.min_p);
+    T vn = u_f * nxs + v_f * nys;
+    T vt = -v_f * nxs + u_f * nys;
+    T T_f = p_f / (rho_f * k.R);
+    T_f = tmin(T_f, k.max_T);
+    T_f = tmax(T_f, k.min_T);
+    p_f = rho_f * k.R * T_f;
+    T ug, vg;
+    if (k.visc) {
+        ug = -u_f; vg = -v_f;
+    } else {
+        T slip = T(0);
+        if (k.wall_mode == 1) slip = T(1);
+        else if (vn > T(0)) slip = tmin(T(0.8f), vn / (vn + k.slip_ref));
+        T us = -vn * nxs + vt * nys;
+        T vs = -vn * nys - vt * nxs;
+        ug = (T(1) - slip) * (-u_f) + slip * us;
+        vg = (T(1) - slip) * (-v_f) + slip * vs;
+    }
+    T vel = ::sqrt(ug * ug + vg * vg);
+    if (vel > k.vghost) {
+        T s = k.vghost / vel;
+        ug *= s; vg *= s;
+    }
+    T Tg;
+    if (k.adiabatic) Tg = T_f;
+    else {
+        Tg = T(2) * k.T_wall - T_f;
+        Tg = tmax(Tg, k.min_T);
+        Tg = tmin(Tg, k.max_T);
+    }
+    T rg = tmax(p_f / (k.R * Tg), k.min_rho);
+    T eg = p_f / k.gm1;
+    T keg = T(0.5) * rg * (ug * ug + vg * vg);
+    o[0] = rg; o[1] = rg * ug; o[2] = rg * vg; o[3] = eg + keg; o[4] = eg;
+}
+#pragma clang fp contract(on)
+
 // Waves per SIMD asked of the register allocator for k_stage: six for FP32
 // (at most 80 VGPRs; measured faster than the five an unconstrained allocation
 // allows) unless the tile's LDS already admits fewer workgroups per CU (160
@@ -283,10 +383,31 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
             int gi = min(max(i0 + li - 2, 0), nx - 1);
             int gj = min(max(j0 + lj - 2, 0), ny - 1);
             size_t g = (size_t)gj * nx + gi;
-            T rho = in0[g];
+            const uint8_t c = ctp[g];
+            T q[5] = {in0[g], in1[g], in2[g], in3[g], in4[g]};
+            if (MODE == 1 && c == CT_GHOST) {
+                // Ghost fill of the stage-1 result, done here instead of in a
+                // launch of its own: ghost neighbours at their pre-BC value
+                // (the blend input), others as stage 1 left them.  The tile that
+                // owns the cell stores it for the later kernels: into the stage-1
+                // set (inviscid: the final ghost fill blends it), or blended into
+                // the output (viscous: the second half step reads it).
+                ghost_value(cd, r.in, r.blend, -1, ghost_index(cd.b, gi, gj), q);
+                const bool own = li >= 2 && li < TX + 2 && lj >= 2 && lj < TY + 2 && i0 + li - 2 < nx && j0 + lj - 2 < ny;
+                if (own) {
+                    if (!visc) {
+                        for (int f = 0; f < 5; f++) ((T*)cd.b.U[r.in][f])[g] = q[f];
+                    } else {
+                        for (int f = 0; f < 5; f++) {
+                            T bl = ((const T*)cd.b.U[r.blend][f])[g];
+                            ((T*)cd.b.U[r.out][f])[g] = T(0.5) * bl + T(0.5) * q[f];
+                        }
+                    }
+                }
+            }
             T u, v, p;
-            primitives_uvp(k, rho, in1[g], in2[g], in3[g], in4[g], u, v, p);
-            s_r[n] = rho; s_u[n] = u; s_v[n] = v; s_p[n] = p; s_ct[n] = ctp[g];
+            primitives_uvp(k, q[0], q[1], q[2], q[3], q[4], u, v, p);
+            s_r[n] = q[0]; s_u[n] = u; s_v[n] = v; s_p[n] = p; s_ct[n] = c;
         }
         __syncthreads();
 
@@ -432,6 +553,12 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
                     }
                 }
             }
+            if (MODE == 1 && c == CT_GHOST) {
+                // stored by the staging loop
+                if (KAHAN)
+                    for (int f = 0; f < 5; f++) kc[f][g] = 0.0f;
+                continue;
+            }
             const bool apply_bc = (MODE == 0) || !visc;
             if (apply_bc) {
                 if (c == CT_GHOST) {
@@ -455,26 +582,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Immersed-boundary ghost fill (applyBoundaryConditionsKernel ghost branch,
-// src/solver.cu:2045-2225) over a list of ghost cells.  A neighbour that is a
-// ghost cell is read at its pre-BC value (set pre0, or the SSP blend of pre0
-// and pre1); any other neighbour at its final value in set out.  Normal and
-// neighbour choice use IEEE arithmetic so the choice matches the oracle.
-// FINAL: also the CFL maxima of the filled cells.
-// ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ T pre_val(const CaseBufs& b, const Roles& r, size_t n, int f) {
-    T a = ((const T*)b.U[r.pre0][f])[n];
-    if (r.pre1 < 0) return a;
-    T c = ((const T*)b.U[r.pre1][f])[n];
-    return T(0.5) * a + T(0.5) * c;
-}
-
 #pragma clang fp contract(off)
-__constant__ int c_ox[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
-__constant__ int c_oy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
-
 // Normal and ranked neighbour list of each ghost cell (static per geometry).
 template <class T>
 __global__ __launch_bounds__(NT) void k_ghost_prep(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ items,
@@ -598,73 +706,9 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
         const CaseConsts<T>& k = cd.c;
         const Roles r = k.visc ? rv : ri;
         const CaseBufs& b = cd.b;
-        const GhostInfo<T> gi = ((const GhostInfo<T>*)b.ghosts)[it.index];
-        const int nx = k.nx;
-        const size_t idx = (size_t)gi.idx;
-        const uint8_t* ct = b.ct;
-        const T nxs = gi.nx, nys = gi.ny;
-        const int cnt = (int)(gi.order >> 24);
-        long long best = -1;
-        bool best_ghost = false;
-        for (int q = 0; q < cnt; q++) {
-            int m = (gi.order >> (3 * q)) & 7;
-            size_t nidx = (size_t)((long long)idx + c_oy[m] * (long long)nx + c_ox[m]);
-            bool g = ct[nidx] == CT_GHOST;
-            T rn = g ? pre_val<T>(b, r, nidx, 0) : ((const T*)b.U[r.out][0])[nidx];
-            if (rn >= k.min_rho) {
-                best = (long long)nidx;
-                best_ghost = g;
-                break;
-            }
-        }
+        const size_t idx = (size_t)((const GhostInfo<T>*)b.ghosts)[it.index].idx;
         T o[5];
-        if (best < 0) {
-            o[0] = k.rho_inf; o[1] = T(0); o[2] = T(0); o[3] = k.e_inf; o[4] = k.e_inf;
-        } else {
-            size_t bn = (size_t)best;
-            T fv[4];
-            for (int f = 0; f < 4; f++) fv[f] = best_ghost ? pre_val<T>(b, r, bn, f) : ((const T*)b.U[r.out][f])[bn];
-            T rho_f = fv[0];
-            T u_f = fv[1] / rho_f, v_f = fv[2] / rho_f;
-            T ke_f = T(0.5) * rho_f * (u_f * u_f + v_f * v_f);
-            T p_f = k.gm1 * (fv[3] - ke_f);
-            rho_f = tmax(rho_f, k.min_rho);
-            p_f = tmax(p_f, k.min_p);
-            T vn = u_f * nxs + v_f * nys;
-            T vt = -v_f * nxs + u_f * nys;
-            T T_f = p_f / (rho_f * k.R);
-            T_f = tmin(T_f, k.max_T);
-            T_f = tmax(T_f, k.min_T);
-            p_f = rho_f * k.R * T_f;
-            T ug, vg;
-            if (k.visc) {
-                ug = -u_f; vg = -v_f;
-            } else {
-                T slip = T(0);
-                if (k.wall_mode == 1) slip = T(1);
-                else if (vn > T(0)) slip = tmin(T(0.8f), vn / (vn + k.slip_ref));
-                T us = -vn * nxs + vt * nys;
-                T vs = -vn * nys - vt * nxs;
-                ug = (T(1) - slip) * (-u_f) + slip * us;
-                vg = (T(1) - slip) * (-v_f) + slip * vs;
-            }
-            T vel = ::sqrt(ug * ug + vg * vg);
-            if (vel > k.vghost) {
-                T s = k.vghost / vel;
-                ug *= s; vg *= s;
-            }
-            T Tg;
-            if (k.adiabatic) Tg = T_f;
-            else {
-                Tg = T(2) * k.T_wall - T_f;
-                Tg = tmax(Tg, k.min_T);
-                Tg = tmin(Tg, k.max_T);
-            }
-            T rg = tmax(p_f / (k.R * Tg), k.min_rho);
-            T eg = p_f / k.gm1;
-            T keg = T(0.5) * rg * (ug * ug + vg * vg);
-            o[0] = rg; o[1] = rg * ug; o[2] = rg * vg; o[3] = eg + keg; o[4] = eg;
-        }
+        ghost_value(cd, r.out, r.pre0, r.pre1, it.index, o);
         if (active)
             for (int f = 0; f < 5; f++) ((T*)b.U[r.out][f])[idx] = o[f];
         if (FINAL) {
@@ -672,7 +716,7 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
             float nu = 0.0f;
             if (active && k.visc) {
                 T pv[5];
-                for (int f = 0; f < 5; f++) pv[f] = pre_val<T>(b, r, idx, f);
+                for (int f = 0; f < 5; f++) pv[f] = pre_val<T>(b, r.pre0, r.pre1, idx, f);
                 T u_, v_, p_, t_;
                 primitives(k, pv[0], pv[1], pv[2], pv[3], pv[4], u_, v_, p_, t_);
                 nu = (float)(sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f)));
