@@ -488,6 +488,8 @@ int fvm_batch_create(int device, size_t mem_budget_bytes, int max_cases, fvm_bat
     if (rc) return rc;
     rc = b->eng->set_tile(FVM_BATCH_TILE_X, FVM_BATCH_TILE_Y);
     if (rc) return rc;
+    rc = b->eng->reserve_cases(max_cases);
+    if (rc) return rc;
     rc = b->eng->set_arena(mem_budget_bytes);
     if (rc) return rc;
     *out = b.release();

@@ -102,10 +102,11 @@ Engine::~Engine() {
         delete c;
     }
     if (d_cases_) (void)hipFree(d_cases_);
-    if (d_tiles_all_) (void)hipFree(d_tiles_all_);
-    if (d_tiles_visc_) (void)hipFree(d_tiles_visc_);
-    if (d_ghosts_) (void)hipFree(d_ghosts_);
-    if (d_case_list_) (void)hipFree(d_case_list_);
+    if (!use_arena_) {   // arena-held lists go with the arena
+        for (WorkItem* p : {d_tiles_all_, d_tiles_visc_, d_ghosts_, d_case_list_, scr_tiles_, scr_ghosts_})
+            if (p) (void)hipFree(p);
+    }
+    if (scr_cnt_) (void)hipFree(scr_cnt_);
     if (d_counts_) (void)hipFree(d_counts_);
     if (d_stat_) (void)hipFree(d_stat_);
     if (d_cons_) (void)hipFree(d_cons_);
@@ -124,11 +125,30 @@ int Engine::init_stream() {
         FVM_HIP(hipMemset(d_counts_, 0, 8 * sizeof(int)));
     }
     if (!d_stat_) FVM_HIP(hipMalloc((void**)&d_stat_, 16 * sizeof(unsigned)));
+    if (!scr_cnt_) FVM_HIP(hipMalloc((void**)&scr_cnt_, 4 * sizeof(int)));
     if (!d_cons_) FVM_HIP(hipMalloc((void**)&d_cons_, (1024 * 4 + 8) * sizeof(double)));
     if (!h_pinned_) FVM_HIP(hipHostMalloc((void**)&h_pinned_, 16 * sizeof(float)));
     for (auto& ts : tset_)
         for (auto& ev : ts.ev)
             if (!ev) FVM_HIP(hipEventCreate(&ev));
+    return FVM_OK;
+}
+
+// Descriptor table for n cases, allocated up front (the batch does this at
+// creation, so admission never grows it).
+int Engine::reserve_cases(int n) {
+    if (n <= max_cases_) return FVM_OK;
+    void* nd = nullptr;
+    FVM_HIP(hipStreamSynchronize(stream_));
+    FVM_HIP(hipMalloc(&nd, desc_size_ * n));
+    if (d_cases_) {
+        FVM_HIP(hipMemcpy(nd, d_cases_, desc_size_ * max_cases_, hipMemcpyDeviceToDevice));
+        FVM_HIP(hipFree(d_cases_));
+    }
+    d_cases_ = nd;
+    h_descs_.resize(desc_size_ * n);
+    max_cases_ = n;
+    drop_graphs();
     return FVM_OK;
 }
 
@@ -397,17 +417,8 @@ int Engine::add_case(const fvm_params& p, int steps_total, bool all_sets, int* i
             // grow the descriptor table (never while a graph references it)
             int nmax = std::max(4, max_cases_ * 2);
             while (nmax <= id) nmax *= 2;
-            void* nd = nullptr;
-            FVM_HIP(hipStreamSynchronize(stream_));
-            FVM_HIP(hipMalloc(&nd, desc_size_ * nmax));
-            if (d_cases_) {
-                FVM_HIP(hipMemcpy(nd, d_cases_, desc_size_ * max_cases_, hipMemcpyDeviceToDevice));
-                FVM_HIP(hipFree(d_cases_));
-            }
-            d_cases_ = nd;
-            h_descs_.resize(desc_size_ * nmax);
-            max_cases_ = nmax;
-            drop_graphs();
+            int rg = reserve_cases(nmax);
+            if (rg) return rg;
         }
         cases_.push_back(nullptr);
     }
@@ -464,19 +475,39 @@ int Engine::remove_case(int id) {
     return FVM_OK;
 }
 
-// Work list of one case's tiles in a scratch device buffer.
-static int upload_items(const std::vector<WorkItem>& v, WorkItem** buf, size_t* cap, int* d_count, int count_slot,
-                        hipStream_t s) {
+// Device memory of the engine's own lists: from the arena when the engine has
+// one (admission and release then never call hipMalloc / hipFree), else
+// hipMalloc.
+void* Engine::dev_alloc(size_t bytes) {
+    if (use_arena_) return arena_.alloc(bytes);
+    void* p = nullptr;
+    return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+}
+
+void Engine::dev_free(void* p) {
+    if (!p) return;
+    if (use_arena_) arena_.release(p);
+    else (void)hipFree(p);
+}
+
+// Uploads a work list into *buf (grown geometrically) and its length into
+// d_count[count_slot].  Stream-ordered: a released buffer is reused only by
+// work queued after everything that read it.
+int Engine::upload_items(const std::vector<WorkItem>& v, WorkItem** buf, size_t* cap, int* d_count, int count_slot) {
     if (v.size() > *cap) {
-        if (*buf) FVM_HIP(hipFree(*buf));
+        FVM_HIP(hipStreamSynchronize(stream_));
+        dev_free(*buf);
+        *buf = nullptr;
+        *cap = 0;
         size_t nc = std::max<size_t>(v.size(), 64) * 2;
-        FVM_HIP(hipMalloc((void**)buf, nc * sizeof(WorkItem)));
+        *buf = (WorkItem*)dev_alloc(nc * sizeof(WorkItem));
+        if (!*buf) return set_error(FVM_ERR_CAPACITY, "device memory exhausted (work list)");
         *cap = nc;
     }
-    if (!v.empty()) FVM_HIP(hipMemcpyAsync(*buf, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+    if (!v.empty()) FVM_HIP(hipMemcpyAsync(*buf, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream_));
     int n = (int)v.size();
-    FVM_HIP(hipMemcpyAsync(d_count + count_slot, &n, sizeof(int), hipMemcpyHostToDevice, s));
-    FVM_HIP(hipStreamSynchronize(s));
+    FVM_HIP(hipMemcpyAsync(d_count + count_slot, &n, sizeof(int), hipMemcpyHostToDevice, stream_));
+    FVM_HIP(hipStreamSynchronize(stream_));
     return FVM_OK;
 }
 
@@ -597,15 +628,13 @@ int Engine::mark_and_lists(HostCase& hc, int fixup) {
     for (int t = 0; t < hc.tiles_x * hc.tiles_y; t++) tiles.push_back(WorkItem{hc.id, t});
     xcd_order(tiles, ordered);
     for (int g = 0; g < (int)hc.ghosts.size(); g++) gl.push_back(WorkItem{hc.id, g});
-    // per-case lists for the reset-path launches
-    WorkItem* d_t = nullptr;
-    WorkItem* d_g = nullptr;
-    size_t ct_cap = 0, cg_cap = 0;
-    int* d_cnt = nullptr;
-    FVM_HIP(hipMalloc((void**)&d_cnt, 4 * sizeof(int)));
-    int rc = upload_items(ordered, &d_t, &ct_cap, d_cnt, 0, stream_);
-    if (!rc) rc = upload_items(gl, &d_g, &cg_cap, d_cnt, 1, stream_);
+    // per-case lists for the reset-path launches (engine scratch, reused)
+    int rc = upload_items(ordered, &scr_tiles_, &scr_tiles_cap_, scr_cnt_, 0);
+    if (!rc) rc = upload_items(gl, &scr_ghosts_, &scr_ghosts_cap_, scr_cnt_, 1);
     if (rc) return rc;
+    WorkItem* const d_t = scr_tiles_;
+    WorkItem* const d_g = scr_ghosts_;
+    int* const d_cnt = scr_cnt_;
 
     Roles r{};
     r.in = cur; r.out = cur; r.blend = -1; r.pre0 = 1; r.pre1 = -1;
@@ -629,9 +658,6 @@ int Engine::mark_and_lists(HostCase& hc, int fixup) {
     launch_tiled(fp64_, tx_, ty_, kc);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(stream_));
-    (void)hipFree(d_t);
-    if (d_g) (void)hipFree(d_g);
-    (void)hipFree(d_cnt);
     lists_dirty_ = true;
     if (!persistent_) drop_graphs();
     return FVM_OK;
@@ -723,10 +749,10 @@ int Engine::rebuild_lists() {
     xcd_order(all, oall);
     oall.insert(oall.begin(), heavy.begin(), heavy.end());
     xcd_order(visc, ovisc);
-    int rc = upload_items(oall, &d_tiles_all_, &cap_tiles_, d_counts_, 0, stream_);
-    if (!rc) rc = upload_items(ovisc, &d_tiles_visc_, &cap_visc_, d_counts_, 1, stream_);
-    if (!rc) rc = upload_items(ghosts, &d_ghosts_, &cap_ghosts_, d_counts_, 2, stream_);
-    if (!rc) rc = upload_items(clist, &d_case_list_, &cap_case_list_, d_counts_, 3, stream_);
+    int rc = upload_items(oall, &d_tiles_all_, &cap_tiles_, d_counts_, 0);
+    if (!rc) rc = upload_items(ovisc, &d_tiles_visc_, &cap_visc_, d_counts_, 1);
+    if (!rc) rc = upload_items(ghosts, &d_ghosts_, &cap_ghosts_, d_counts_, 2);
+    if (!rc) rc = upload_items(clist, &d_case_list_, &cap_case_list_, d_counts_, 3);
     if (rc) return rc;
     n_tiles_ = (int)oall.size();
     n_visc_ = (int)ovisc.size();
@@ -1006,12 +1032,10 @@ int Engine::write_state(int id, const void* host) {
     std::vector<WorkItem> tiles, ordered;
     for (int t = 0; t < hc->tiles_x * hc->tiles_y; t++) tiles.push_back(WorkItem{hc->id, t});
     xcd_order(tiles, ordered);
-    WorkItem* d_t = nullptr;
-    size_t cap = 0;
-    int* d_cnt = nullptr;
-    FVM_HIP(hipMalloc((void**)&d_cnt, 4 * sizeof(int)));
-    int rc = upload_items(ordered, &d_t, &cap, d_cnt, 0, stream_);
+    int rc = upload_items(ordered, &scr_tiles_, &scr_tiles_cap_, scr_cnt_, 0);
     if (rc) return rc;
+    WorkItem* const d_t = scr_tiles_;
+    int* const d_cnt = scr_cnt_;
     FVM_HIP(hipMemsetAsync((char*)hc->mem + L.slots_off + phase_ * 8, 0, 8, stream_));
     if (kahan_) FVM_HIP(hipMemsetAsync((char*)hc->mem + L.k_off, 0, round256(L.n * 4) * 5, stream_));
     Roles r{cur, cur, -1, -1, -1, phase_, phase_, 0, 0};
@@ -1027,8 +1051,6 @@ int Engine::write_state(int id, const void* host) {
     launch_tiled(fp64_, tx_, ty_, kc);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(stream_));
-    (void)hipFree(d_t);
-    (void)hipFree(d_cnt);
     return FVM_OK;
 }
 

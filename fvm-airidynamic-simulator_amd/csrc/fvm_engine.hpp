@@ -120,6 +120,7 @@ public:
     int retire_finished();
     size_t bytes_in_use() const;
     int set_arena(size_t bytes);
+    int reserve_cases(int n);
     bool fp64() const { return fp64_; }
     bool kahan() const { return kahan_; }
     hipStream_t stream() const { return stream_; }
@@ -140,6 +141,9 @@ private:
     int enqueue_step(int phase);
     int capture_graphs();
     void drop_graphs();
+    void* dev_alloc(size_t bytes);
+    void dev_free(void* p);
+    int upload_items(const std::vector<WorkItem>& v, WorkItem** buf, size_t* cap, int* d_count, int count_slot);
     int timed_begin();
     int timed(const char* name, int rc_before);
     int flush_timed_set(int set);
@@ -158,6 +162,10 @@ private:
     WorkItem* d_tiles_visc_ = nullptr;
     WorkItem* d_ghosts_ = nullptr;
     WorkItem* d_case_list_ = nullptr;   // active cases, for the per-tile maxima reduction
+    WorkItem* scr_tiles_ = nullptr;     // one case's lists for the reset-path launches
+    WorkItem* scr_ghosts_ = nullptr;
+    int* scr_cnt_ = nullptr;
+    size_t scr_tiles_cap_ = 0, scr_ghosts_cap_ = 0;
     int* d_counts_ = nullptr;        // [0] all tiles, [1] viscous tiles, [2] ghosts, [3] active cases
     size_t cap_tiles_ = 0, cap_visc_ = 0, cap_ghosts_ = 0, cap_case_list_ = 0;
     int n_tiles_ = 0, n_visc_ = 0, n_ghosts_ = 0;
