@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--overload-rate", type=float, default=50.0, help="requests/s of the short overload stream")
     ap.add_argument("--overload-seconds", type=float, default=4.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-other", action="store_true", help="skip the 256x128 Euler case and the CPU mix subset")
     return ap.parse_args()
 
 
@@ -89,6 +90,70 @@ def cpu_baseline(p, nsteps):
     rate = p.nx * p.ny * nsteps / dt
     return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{nsteps} steps of the {p.nx}x{p.ny} viscous workload, FP32 oracle, OpenMP, {dt:.1f} s"}
+
+
+def cpu_rate(p, nsteps, threads):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    O.set_threads(threads)
+    q = p.copy()
+    o = O.Oracle(q, fp64=False)
+    t0 = time.perf_counter()
+    o.step(nsteps, q)
+    dt = time.perf_counter() - t0
+    o.close()
+    return p.nx * p.ny * nsteps / dt, dt
+
+
+def other_configs(with_cpu):
+    """The other baseline rows: the 256x128 Mach-3 Euler case (launch-latency
+    bound on the GPU) in full, 1000 steps, on the GPU and on the CPU oracle;
+    and a 32-case x 200-step subset of the batch mix on the CPU oracle."""
+    import fvm_amd
+    from fvm_amd import params as P
+    out = {}
+    p = P.euler_bow_shock()
+    p.dt_mode = P.DT_DEVICE_CFL
+    s = fvm_amd.CFDSolver(0)
+    s.initialize(p)
+    q = p.copy()
+    s.advance(q, 100)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.advance(q, 1000)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    s.close()
+    out["euler_256x128"] = {"gpu_cell_updates_per_s": p.nx * p.ny * 1000 / dt, "gpu_us_per_step": dt / 1000 * 1e6}
+    # the shape of the reference's README screenshot run (SI Mach-3 circle r = 0.5 m
+    # in a 10 x 5 m domain on 1486 x 1828 cells, inviscid, CFL 0.5): context only
+    r = P.default_params()
+    r.nx, r.ny = 1486, 1828
+    P.compute_derived(r)
+    r.dt_mode = P.DT_DEVICE_CFL
+    s = fvm_amd.CFDSolver(0)
+    s.initialize(r)
+    q = r.copy()
+    s.advance(q, 20)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.advance(q, 200)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    s.close()
+    out["si_mach3_1486x1828"] = {"gpu_cell_updates_per_s": r.nx * r.ny * 200 / dt, "gpu_ms_per_step": dt / 200 * 1e3}
+    if with_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        rate, sec = cpu_rate(p, 1000, threads)
+        out["euler_256x128"].update({"cpu_cell_updates_per_s": rate, "cpu_seconds": sec, "cpu_threads": threads})
+        cases = P.mixed_batch(512)[:32]
+        cells = tot = 0.0
+        for c in cases:
+            r, sec = cpu_rate(c, 200, threads)
+            cells += c.nx * c.ny * 200
+            tot += sec
+        out["mix_32x200_cpu"] = {"cpu_cell_updates_per_s": cells / tot, "cpu_seconds": tot, "cpu_threads": threads}
+    return out
 
 
 def batched_vs_plain(ncases, nsteps):
@@ -220,6 +285,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(p, a.cpu_steps)
     else:
         out["cpu_baseline"] = None
+    if not a.no_other:
+        out["other_configs"] = other_configs(not a.no_cpu)
     print(json.dumps(out), flush=True)
 
 
