@@ -396,6 +396,12 @@ static bool params_valid(const fvm_params& p, std::string& why) {
     if (p.limiter < 0 || p.limiter > 1) { why = "unknown limiter"; return false; }
     if (p.dt_mode < 0 || p.dt_mode > 1) { why = "unknown dt mode"; return false; }
     if (p.precision < 0 || p.precision > 2) { why = "unknown precision"; return false; }
+    if (p.wall_mode < 0 || p.wall_mode > 1) { why = "unknown wall mode"; return false; }
+    if (p.noise_field < 0 || p.noise_field > 2) { why = "unknown noise field"; return false; }
+    if (p.obstacle_shape < 0 || p.obstacle_shape > 5) { why = "unknown obstacle shape"; return false; }
+    if (!(p.cfl > 0) || (p.enable_viscosity && !(p.cfl_visc > 0))) { why = "CFL numbers must be positive"; return false; }
+    if (p.dt_mode == FVM_DT_HOST && !(p.dt > 0)) { why = "dt must be positive with host dt"; return false; }
+    if (!(p.min_density > 0) || !(p.min_pressure > 0)) { why = "density and pressure floors must be positive"; return false; }
     if (p.precision == FVM_PRECISION_FP32_KAHAN && p.enable_viscosity) {
         why = "the compensated (Kahan) update is implemented for inviscid cases only";
         return false;

@@ -255,3 +255,17 @@ def test_obstacle_update_mid_run(solver, visc):
     for l1, _ in e0 + e1:
         assert l1 < L1_BAR
     r.close()
+
+
+@pytest.mark.parametrize("field,value", [("nx", 2), ("flux_scheme", 7), ("wall_mode", 3), ("cfl", 0.0),
+                                         ("obstacle_shape", 9), ("dt", -1.0)])
+def test_invalid_params_rejected(solver, field, value):
+    """Bad parameters come back as FVM_ERR_INVALID_ARGUMENT with a message, no launch."""
+    p = P.euler_bow_shock(nx=64, ny=32, radius=6.0, center=(16.0, 16.0))
+    p.dt = 0.1
+    setattr(p, field, value)
+    announce(f"invalid params {field}={value}")
+    with pytest.raises(fvm_amd.FvmError) as e:
+        solver.initialize(p)
+    assert e.value.code == -1
+    solver.initialize(P.euler_bow_shock())   # the handle stays usable
