@@ -328,3 +328,50 @@ def test_large_grid_freestream():
     t, steps = s.time()
     assert steps == 3 and t == pytest.approx(0.3, rel=1e-6)
     s.close()
+
+
+def test_c1_configuration_1000_steps(solver):
+    """The 256x128 Mach-3 Euler configuration as specified: AUSM+ and minmod,
+    CFL 0.5 with the CFL dt recomputed every step, density noise, 1000 steps,
+    against the FP64 oracle (FP32 rounding alone, the FP32 oracle against the
+    FP64 one, gives at most 6.1e-5)."""
+    p = P.euler_bow_shock(flux=P.FLUX_AUSM_PLUS, limiter=P.LIMITER_MINMOD)
+    p.dt_mode = P.DT_DEVICE_CFL
+    announce("C1 configuration 256x128, AUSM+ minmod, device CFL, 1000 steps")
+    solver.initialize(p)
+    solver.advance(p.copy(), 1000)
+    o = O.Oracle(p, fp64=True)
+    o.step(1000, p.copy())
+    errs = rel_errors(solver.state64(), o.state())
+    print("  rel L1:", ["%.2e" % e[0] for e in errs], " Linf:", ["%.2e" % e[1] for e in errs], flush=True)
+    for l1, _ in errs:
+        assert l1 < L1_BAR
+    assert solver.time()[0] == pytest.approx(o.time()[0], rel=1e-5)
+    o.close()
+
+
+def test_c2_full_size_20_steps(solver):
+    """The headline configuration at its full size: 2048x1024 viscous cylinder
+    (D 64 at (512, 512), Re 200, Mach 0.3, HLLC + van Leer, device CFL dt),
+    20 steps against the FP64 oracle.  At step 20 the transverse momentum is
+    still noise-sized (sum|rho_v| = 3e-4 sum|rho_u|), so both momentum
+    components are measured on the momentum scale: FP32 rounding alone (the
+    FP32 oracle against the FP64 one) gives 1e-7 there but 3.2e-4 of rho_v's
+    own sum."""
+    p = P.viscous_cylinder(nx=2048, ny=1024, diameter=64.0, reynolds=200.0, mach=0.3, center=(512.0, 512.0))
+    p.dt_mode = P.DT_DEVICE_CFL
+    announce("C2 configuration 2048x1024 viscous, device CFL, 20 steps")
+    solver.initialize(p)
+    solver.advance(p.copy(), 20)
+    o = O.Oracle(p, fp64=True)
+    o.step(20, p.copy())
+    assert np.array_equal(solver.getCellTypes(), o.cell_types())
+    g, r = solver.state64(), o.state()
+    errs = rel_errors(g, r)
+    mom = np.abs(r[1]).sum() + np.abs(r[2]).sum()
+    scaled = [np.abs(g[c] - r[c]).sum() / (mom if c in (1, 2) else np.abs(r[c]).sum()) for c in range(5)]
+    print("  rel L1:", ["%.2e" % e[0] for e in errs], " Linf:", ["%.2e" % e[1] for e in errs],
+          " momentum-scaled:", ["%.2e" % e for e in scaled], flush=True)
+    for e in scaled:
+        assert e < L1_BAR
+    o.close()
