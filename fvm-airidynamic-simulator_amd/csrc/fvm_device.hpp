@@ -171,16 +171,22 @@ __device__ __forceinline__ void primitives_uvp(const CaseConsts<T>& k, T rho, T 
     p = tmax((T)k.gm1 * e, (T)k.min_p);
 }
 
-// minmod (solver.cu:110-115) and van Leer limited slopes
+// Half of the limited MUSCL slope, i.e. the offset of a face value from the
+// cell value: minmod (solver.cu:110-115) or van Leer 2ab/(a+b).  Returned
+// halved so that van Leer needs no doubling; halving is exact, so the face
+// values are the same as from the full slope times 1/2.
 template <class T>
-__device__ __forceinline__ T limited_slope(int limiter, T dL, T dR) {
+__device__ __forceinline__ T half_limited_slope(int limiter, T dL, T dR) {
     if (limiter == 1) {
         T prod = dL * dR;
-        return prod > T(0) ? Num<T>::div(T(2) * prod, dL + dR) : T(0);
+        return prod > T(0) ? Num<T>::div(prod, dL + dR) : T(0);
     }
     T m = tmin(tabs(dL), tabs(dR));
-    return (dL * dR > T(0)) ? copysign(m, dL) : T(0);
+    return (dL * dR > T(0)) ? T(0.5) * copysign(m, dL) : T(0);
 }
+
+// Solid or ghost tag (CT_SOLID = 1, CT_GHOST = 2) in one unsigned compare.
+__device__ __forceinline__ bool wall_tag(uint8_t c) { return (unsigned)c - 1u <= 1u; }
 
 // Physical flux in face-normal form (computeFluxX, solver.cu:1005-1014).
 // pg: gauge pressure removed from the momentum flux before rounding.
@@ -295,22 +301,22 @@ __device__ __forceinline__ void face_flux(const CaseConsts<T>& k, const T rho[4]
     // gauge pressure (the freestream pressure): a constant added to every face
     // flux cancels in the divergence, and the face pressures ~p_inf no longer
     // swamp the small momentum changes of low-Mach flow in FP32 rounding.
-    auto sg = [](uint8_t c) { return c == CT_SOLID || c == CT_GHOST; };
-    if (sg(ct[1]) || sg(ct[2])) {
+    if (wall_tag(ct[1]) || wall_tag(ct[2])) {
         out[0] = T(0); out[1] = p[1] - k.p_gauge; out[2] = T(0); out[3] = T(0); out[4] = T(0);
         return;
     }
-    bool first = sg(ct[0]) || sg(ct[3]) || rho[1] < (T)k.low_rho || rho[2] < (T)k.low_rho;
-    T h = first ? T(0) : T(0.5);
+    // second order unless a +-2 neighbour is a wall or a face cell is near vacuum
+    bool first = wall_tag(ct[0]) || wall_tag(ct[3]) || rho[1] < (T)k.low_rho || rho[2] < (T)k.low_rho;
+    T h = first ? T(0) : T(1);
     int lim = k.limiter;
-    T rL = rho[1] + h * limited_slope(lim, rho[1] - rho[0], rho[2] - rho[1]);
-    T uL = un[1] + h * limited_slope(lim, un[1] - un[0], un[2] - un[1]);
-    T vL = ut[1] + h * limited_slope(lim, ut[1] - ut[0], ut[2] - ut[1]);
-    T pL = p[1] + h * limited_slope(lim, p[1] - p[0], p[2] - p[1]);
-    T rR = rho[2] - h * limited_slope(lim, rho[2] - rho[1], rho[3] - rho[2]);
-    T uR = un[2] - h * limited_slope(lim, un[2] - un[1], un[3] - un[2]);
-    T vR = ut[2] - h * limited_slope(lim, ut[2] - ut[1], ut[3] - ut[2]);
-    T pR = p[2] - h * limited_slope(lim, p[2] - p[1], p[3] - p[2]);
+    T rL = rho[1] + h * half_limited_slope(lim, rho[1] - rho[0], rho[2] - rho[1]);
+    T uL = un[1] + h * half_limited_slope(lim, un[1] - un[0], un[2] - un[1]);
+    T vL = ut[1] + h * half_limited_slope(lim, ut[1] - ut[0], ut[2] - ut[1]);
+    T pL = p[1] + h * half_limited_slope(lim, p[1] - p[0], p[2] - p[1]);
+    T rR = rho[2] - h * half_limited_slope(lim, rho[2] - rho[1], rho[3] - rho[2]);
+    T uR = un[2] - h * half_limited_slope(lim, un[2] - un[1], un[3] - un[2]);
+    T vR = ut[2] - h * half_limited_slope(lim, ut[2] - ut[1], ut[3] - ut[2]);
+    T pR = p[2] - h * half_limited_slope(lim, p[2] - p[1], p[3] - p[2]);
     rL = tmax(rL, (T)k.min_rho);
     pL = tmax(pL, (T)k.min_p);
     rR = tmax(rR, (T)k.min_rho);
