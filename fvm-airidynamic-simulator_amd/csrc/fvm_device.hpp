@@ -205,16 +205,16 @@ __device__ __forceinline__ void phys_flux(T rho, T un, T ut, T p, T E, T f[4], T
 template <class T>
 __device__ __forceinline__ void hllc(const CaseConsts<T>& k, T rL, T uL, T vL, T pL, T EL,
                                      T rR, T uR, T vR, T pR, T ER, T f[4]) {
-    // one reciprocal square root per side gives 1/rho and sqrt(rho)
+    // one reciprocal square root per side gives sqrt(rho) = rho rsqrt(rho);
+    // the Roe weight of H = (E + p)/rho is then (E + p) rsqrt(rho)
     T rsL = Num<T>::rsqrt(rL), rsR = Num<T>::rsqrt(rR);
-    T irL = rsL * rsL, irR = rsR * rsR;
     T sL = rL * rsL, sR = rR * rsR;
     T cL = Num<T>::sqrt((T)k.gamma * pL) * rsL;
     T cR = Num<T>::sqrt((T)k.gamma * pR) * rsR;
     T id = Num<T>::rcp(sL + sR + T(1e-10f));
     T u_roe = (sL * uL + sR * uR) * id;
     T v_roe = (sL * vL + sR * vR) * id;
-    T H_roe = (sL * ((EL + pL) * irL) + sR * ((ER + pR) * irR)) * id;
+    T H_roe = ((EL + pL) * rsL + (ER + pR) * rsR) * id;
     T c2 = (T)k.gm1 * (H_roe - T(0.5) * (u_roe * u_roe + v_roe * v_roe));
     T c_roe = (c2 > T(0)) ? Num<T>::sqrt(c2) : T(0.5) * (cL + cR);
     T SL = tmin(uL - cL, u_roe - c_roe);
@@ -240,9 +240,10 @@ __device__ __forceinline__ void hllc(const CaseConsts<T>& k, T rL, T uL, T vL, T
         T rs = tmax(rK * dsk * icd, (T)k.min_rho);
         T ps = tmax(pK + rK * dsk * (S_star - uK), (T)k.min_p);
         T Es = (EK * dsk + ps * S_star - pK * uK) * icd;
-        f[0] += SK * (rs - rK);
+        T dm = SK * (rs - rK);
+        f[0] += dm;
         f[1] += SK * (rs * S_star - rK * uK);
-        f[2] += SK * (rs * vK - rK * vK);
+        f[2] += dm * vK;
         f[3] += SK * (Es - EK);
     }
 }
