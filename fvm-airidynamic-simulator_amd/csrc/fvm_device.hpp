@@ -439,7 +439,8 @@ __device__ __forceinline__ float dt_from_slot(const CaseConsts<T>& k, const unsi
 // atomic max on the bit pattern (non-negative floats order like their bits).
 // The slot only grows during a kernel, so a workgroup whose maximum does not
 // exceed the value already there skips the atomic: thousands of workgroups
-// would otherwise serialise on one address.
+// would otherwise serialise on one address.  Used off the step (maxima after a
+// reset or a state upload, statistics); the step itself stores per-tile maxima.
 __device__ __forceinline__ void slot_max(unsigned* target, float m) {
     if (!(m > 0.0f)) return;
     unsigned cur = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -4,10 +4,12 @@
 // of (case, tile) items so that one launch serves one case or hundreds:
 //   k_diffuse<final=false>   viscous half step D(dt/2) on U^n  -> U_d        (viscous only)
 //   k_stage<STAGE1>          prims + MUSCL + Riemann flux + FV update + point-wise BC -> U*
-//   k_ghost                  immersed-boundary ghost fill of U*
-//   k_stage<STAGE2>          second stage + SSP blend (+ point-wise BC + CFL maxima when inviscid)
-//   k_diffuse<final=true>    second half step + point-wise BC + CFL maxima (viscous only)
-//   k_ghost<final>           ghost fill + CFL maxima of the ghost cells
+//   k_stage<STAGE2>          immersed-boundary ghost fill of U* as the tile is staged,
+//                            second stage + SSP blend (+ point-wise BC + per-tile CFL
+//                            maxima when inviscid)
+//   k_diffuse<final=true>    second half step + point-wise BC + per-tile CFL maxima (viscous only)
+//   k_ghost<final>           ghost fill of the new state, CFL maxima of the ghost cells and
+//                            the reduction of every case's per-tile maxima into its dt slot
 // Each stage kernel stages its tile plus a 2-cell halo through LDS, derives
 // the primitives once per cell, computes every face flux once per tile and
 // keeps flux divergence in registers; no face array ever reaches HBM.
