@@ -142,6 +142,25 @@ def other_configs(with_cpu):
     dt = time.perf_counter() - t0
     s.close()
     out["si_mach3_1486x1828"] = {"gpu_cell_updates_per_s": r.nx * r.ny * 200 / dt, "gpu_ms_per_step": dt / 200 * 1e3}
+    # host round trip every step (fvm_set_state, one step, fvm_get_state) on the
+    # headline grid: the rate when the caller keeps the state in host memory
+    h = P.viscous_cylinder(nx=2048, ny=1024, diameter=64.0, reynolds=200.0, mach=0.3, center=(512.0, 512.0))
+    h.dt_mode = P.DT_DEVICE_CFL
+    s = fvm_amd.CFDSolver(0)
+    s.initialize(h)
+    q = h.copy()
+    st = s.state()
+    s.advance(q, 2)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        s.set_state(st)
+        s.advance(q, 1)
+        st = s.state()
+    dt = time.perf_counter() - t0
+    s.close()
+    out["host_roundtrip_2048x1024"] = {"cell_updates_per_s": h.nx * h.ny * 10 / dt, "ms_per_step": dt / 10 * 1e3,
+                                       "note": "state uploaded and read back (2 x 40 MB) around every step"}
     if with_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
         rate, sec = cpu_rate(p, 1000, threads)

@@ -7,6 +7,7 @@ known answers, brute-force circle masks, geometric facts of each shape,
 freestream preservation, the CFL closed form and a viscous shear identity.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -354,3 +355,19 @@ def test_obstacle_update_zeroes_exposed_cells():
     assert np.array_equal(zero, exposed)
     o.close()
     fresh.close()
+
+
+def test_openmp_oracle_is_thread_count_independent():
+    """Jacobi boundary semantics and per-cell arithmetic make the oracle's
+    result independent of how cells are split over threads."""
+    p = P.euler_bow_shock(nx=96, ny=48, radius=8.0, center=(30.0, 24.0))
+    p.dt = 0.1
+    out = []
+    for threads in (1, 3, 8):
+        O.set_threads(threads)
+        o = O.Oracle(p, fp64=False)
+        o.step(30, p.copy())
+        out.append(o.state())
+        o.close()
+    O.set_threads(os.cpu_count() or 1)
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
