@@ -72,8 +72,11 @@ def time_batch(cases, tile, steps: int) -> float:
 
 
 def bucket_of(cells: int) -> int:
-    """log4 bucket of the cell count, clamped to 0..4 (32K, 128K, 512K, 2M, 8M+)."""
-    return max(0, min(4, int(round(math.log(max(cells, 1) / 32768.0, 4)))))
+    """log4 bucket of the cell count, clamped to 0..4 (32K, 128K, 512K, 2M, 8M+);
+    halves round away from zero, as the library's lookup (std::lround) does."""
+    x = math.log(max(cells, 1) / 32768.0) / math.log(4.0)
+    r = math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+    return max(0, min(4, int(r)))
 
 
 def main(argv=None):

@@ -173,3 +173,25 @@ int main(void) {
     assert r.returncode in (0, 3), r.stdout + r.stderr
     if r.returncode == 3:
         assert "no device" in r.stdout
+
+
+def test_tuned_tile_table_uses_compiled_shapes():
+    """Every shape the measured table (csrc/tile_table.inc) selects, and the
+    batch default, is one the kernels are compiled for; the Python bucketing
+    used to label the measurements agrees with the library's lookup."""
+    csrc = os.path.join(REPO, "fvm-airidynamic-simulator_amd", "csrc")
+    kern = open(os.path.join(csrc, "fvm_kernels.hip")).read()
+    m = re.search(r"#define FVM_TILE_TABLE\(X\)(.*)", kern)
+    compiled = {tuple(int(v) for v in x) for x in re.findall(r"X\((\d+),\s*(\d+)\)", m.group(1))}
+    inc = open(os.path.join(csrc, "tile_table.inc")).read()
+    entries = [tuple(int(v) for v in e) for e in
+               re.findall(r"\{(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\}", re.search(r"#define FVM_TILE_ENTRIES (.*)", inc).group(1))]
+    assert entries
+    for visc, bucket, tx, ty in entries:
+        assert visc in (0, 1) and 0 <= bucket <= 4 and (tx, ty) in compiled
+    bx = int(re.search(r"#define FVM_BATCH_TILE_X (\d+)", inc).group(1))
+    by = int(re.search(r"#define FVM_BATCH_TILE_Y (\d+)", inc).group(1))
+    assert (bx, by) in compiled
+    from fvm_amd.autotune import bucket_of
+    assert [bucket_of(32768 * 4 ** k) for k in range(5)] == [0, 1, 2, 3, 4]
+    assert bucket_of(256 * 256) == 1 and bucket_of(64) == 0 and bucket_of(1 << 30) == 4   # ties round up
