@@ -375,3 +375,47 @@ def test_c2_full_size_20_steps(solver):
     for e in scaled:
         assert e < L1_BAR
     o.close()
+
+
+def reference_units_case(kind):
+    """The reference's own SI configuration (fvm_params_default: Mach 3, T 300 K,
+    p 101325 Pa, circle r 0.5 m in a 10 x 5 m domain) on 256x128 cells, and
+    variants of it: viscous (the KernelBenchmark case), pure-slip wall on the
+    star, isothermal 500 K wall on a rotated capsule, the starship shape."""
+    p = P.default_params()
+    p.nx, p.ny = 256, 128
+    if kind == "si_viscous":
+        p.enable_viscosity = 1
+    elif kind == "slip_star":
+        p.obstacle_shape, p.wall_mode = P.SHAPE_STAR, P.WALL_SLIP
+    elif kind == "isothermal_capsule":
+        p.enable_viscosity, p.adiabatic_wall, p.T_wall = 1, 0, 500.0
+        p.obstacle_shape, p.obstacle_rotation = P.SHAPE_CAPSULE, 0.4
+    elif kind == "starship":
+        p.obstacle_shape, p.wing_rotation, p.obstacle_r = P.SHAPE_STARSHIP, 0.3, 0.8
+    P.compute_derived(p)
+    return p
+
+
+@pytest.mark.parametrize("kind", ["si_inviscid", "si_viscous", "slip_star", "isothermal_capsule", "starship"])
+def test_reference_si_configurations(solver, kind):
+    """100 steps at the dt computeStableTimeStep gives once (the reference's
+    front end sets dt like this), against the FP64 oracle; momentum on the
+    momentum scale.  FP32 rounding alone (FP32 oracle against FP64) gives at
+    most 3.5e-5 here (starship), 3e-7 for the others."""
+    p = reference_units_case(kind)
+    o = O.Oracle(p, fp64=True)
+    p.dt = o.stable_dt(p)
+    announce(f"SI configuration {kind} 256x128, dt {p.dt:.3e}, 100 steps")
+    solver.initialize(p)
+    assert np.array_equal(solver.getCellTypes(), o.cell_types())
+    assert solver.computeStableTimeStep(p) == pytest.approx(p.dt, rel=1e-5)
+    solver.advance(p.copy(), 100)
+    o.step(100, p.copy())
+    g, r = solver.state64(), o.state()
+    mom = np.abs(r[1]).sum() + np.abs(r[2]).sum()
+    errs = [np.abs(g[c] - r[c]).sum() / (mom if c in (1, 2) else np.abs(r[c]).sum()) for c in range(5)]
+    print("  scaled rel L1:", ["%.2e" % e for e in errs], flush=True)
+    for e in errs:
+        assert e < L1_BAR
+    o.close()
