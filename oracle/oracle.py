@@ -1,8 +1,9 @@
 """ctypes binding of the CPU oracle (liboracle.so built from fvm_oracle.cpp).
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker.  The product library never
-loads it.
+cpu_baseline leg of bench.py, always as the reference the GPU results are
+compared with (or, in bench.py, timed as the CPU baseline).  The product
+library never loads it.
 """
 from __future__ import annotations
 
