@@ -137,6 +137,26 @@ template <class T> __device__ __forceinline__ T tmin(T a, T b);
 template <> __device__ __forceinline__ float tmin<float>(float a, float b) { return fminf(a, b); }
 template <> __device__ __forceinline__ double tmin<double>(double a, double b) { return fmin(a, b); }
 
+// One field plane (or any per-cell array) in HBM, indexed by cell number.
+// The case table in device memory holds the buffer pointers, so the compiler
+// sees them as generic pointers and would emit flat loads and stores; flat
+// accesses count on the LDS counter as well (a wait for an LDS read then also
+// waits for outstanding HBM loads) and each needs a 64-bit address.  Plane casts
+// the pointer to the global address space (the address is unchanged) and
+// indexes with a 32-bit byte offset, which gives global_load / global_store
+// with the plane's base in SGPRs and one offset VGPR shared by every field of a
+// cell.  The byte offset must fit in 32 bits: the engine rejects grids whose
+// plane exceeds 4 GiB.
+template <class T> using gmem = __attribute__((address_space(1))) T;
+
+template <class T>
+struct Plane {
+    gmem<char>* base;
+    __device__ __forceinline__ explicit Plane(const void* p) : base((gmem<char>*)p) {}
+    __device__ __forceinline__ T operator[](unsigned i) const { return *(const gmem<T>*)(base + i * (unsigned)sizeof(T)); }
+    __device__ __forceinline__ void put(unsigned i, T v) const { *(gmem<T>*)(base + i * (unsigned)sizeof(T)) = v; }
+};
+
 template <class T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
 template <> __device__ __forceinline__ float tabs<float>(float a) { return fabsf(a); }
 template <> __device__ __forceinline__ double tabs<double>(double a) { return fabs(a); }

@@ -389,6 +389,11 @@ int Engine::upload_desc(HostCase& hc) {
 static bool params_valid(const fvm_params& p, std::string& why) {
     if (p.nx < 3 || p.ny < 3) { why = "nx and ny must be >= 3"; return false; }
     if ((long long)p.nx * p.ny > (1ll << 31) - 1) { why = "grid too large (nx*ny must fit in int32)"; return false; }
+    // the kernels address a field plane with a 32-bit byte offset
+    if ((long long)p.nx * p.ny * (p.precision == FVM_PRECISION_FP64 ? 8 : 4) > (1ll << 32)) {
+        why = "grid too large (one field plane, nx*ny*element bytes, must fit in 4 GiB)";
+        return false;
+    }
     if (!(p.dx > 0) || !(p.dy > 0)) { why = "dx and dy must be positive"; return false; }
     if (!(p.gamma > 1.0f)) { why = "gamma must exceed 1"; return false; }
     if (!(p.rho_inf > 0) || !(p.p_inf > 0)) { why = "rho_inf and p_inf must be positive (call fvm_params_compute_derived)"; return false; }

@@ -240,10 +240,10 @@ __constant__ int c_ox[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
 __constant__ int c_oy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
 
 template <class T>
-__device__ __forceinline__ T pre_val(const CaseBufs& b, int pre0, int pre1, size_t n, int f) {
-    T a = ((const T*)b.U[pre0][f])[n];
+__device__ __forceinline__ T pre_val(const CaseBufs& b, int pre0, int pre1, unsigned n, int f) {
+    T a = Plane<T>(b.U[pre0][f])[n];
     if (pre1 < 0) return a;
-    T c = ((const T*)b.U[pre1][f])[n];
+    T c = Plane<T>(b.U[pre1][f])[n];
     return T(0.5) * a + T(0.5) * c;
 }
 
@@ -251,7 +251,7 @@ __device__ __forceinline__ T pre_val(const CaseBufs& b, int pre0, int pre1, size
 __device__ __forceinline__ int ghost_index(const CaseBufs& b, int i, int j) {
     const int li = i - b.gbox[0], lj = j - b.gbox[1];
     if (li < 0 || lj < 0 || li >= b.gbox[2] || lj >= b.gbox[3]) return -1;
-    return b.gmap[lj * b.gbox[2] + li];
+    return Plane<int>(b.gmap)[(unsigned)(lj * b.gbox[2] + li)];
 }
 
 template <class T>
@@ -260,19 +260,19 @@ __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, 
     const CaseBufs& b = cd.b;
     const GhostInfo<T> gi = ((const GhostInfo<T>*)b.ghosts)[gidx];
     const int nx = k.nx;
-    const size_t idx = (size_t)gi.idx;
-    const uint8_t* ct = b.ct;
+    const int idx = gi.idx;
+    const Plane<uint8_t> ct(b.ct);
     const T nxs = gi.nx, nys = gi.ny;
     const int cnt = (int)(gi.order >> 24);
-    long long best = -1;
+    int best = -1;
     bool best_ghost = false;
     for (int q = 0; q < cnt; q++) {
         int m = (gi.order >> (3 * q)) & 7;
-        size_t nidx = (size_t)((long long)idx + c_oy[m] * (long long)nx + c_ox[m]);
+        unsigned nidx = (unsigned)(idx + c_oy[m] * nx + c_ox[m]);
         bool g = ct[nidx] == CT_GHOST;
-        T rn = g ? pre_val<T>(b, pre0, pre1, nidx, 0) : ((const T*)b.U[src][0])[nidx];
+        T rn = g ? pre_val<T>(b, pre0, pre1, nidx, 0) : Plane<T>(b.U[src][0])[nidx];
         if (rn >= k.min_rho) {
-            best = (long long)nidx;
+            best = (int)nidx;
             best_ghost = g;
             break;
         }
@@ -281,9 +281,9 @@ __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, 
         o[0] = k.rho_inf; o[1] = T(0); o[2] = T(0); o[3] = k.e_inf; o[4] = k.e_inf;
         return;
     }
-    size_t bn = (size_t)best;
+    unsigned bn = (unsigned)best;
     T fv[4];
-    for (int f = 0; f < 4; f++) fv[f] = best_ghost ? pre_val<T>(b, pre0, pre1, bn, f) : ((const T*)b.U[src][f])[bn];
+    for (int f = 0; f < 4; f++) fv[f] = best_ghost ? pre_val<T>(b, pre0, pre1, bn, f) : Plane<T>(b.U[src][f])[bn];
     T rho_f = fv[0];
     T u_f = fv[1] / rho_f, v_f = fv[2] / rho_f;
     T ke_f = T(0.5) * rho_f * (u_f * u_f + v_f * v_f);
@@ -372,19 +372,16 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
             if (r.count_step) { cd.b.t[0] += (double)dtf; cd.b.steps[0] += 1; }
         }
 
-        const T* __restrict__ in0 = (const T*)cd.b.U[r.in][0];
-        const T* __restrict__ in1 = (const T*)cd.b.U[r.in][1];
-        const T* __restrict__ in2 = (const T*)cd.b.U[r.in][2];
-        const T* __restrict__ in3 = (const T*)cd.b.U[r.in][3];
-        const T* __restrict__ in4 = (const T*)cd.b.U[r.in][4];
-        const uint8_t* __restrict__ ctp = cd.b.ct;
+        const Plane<T> in0(cd.b.U[r.in][0]), in1(cd.b.U[r.in][1]), in2(cd.b.U[r.in][2]), in3(cd.b.U[r.in][3]),
+            in4(cd.b.U[r.in][4]);
+        const Plane<uint8_t> ctp(cd.b.ct);
 
         // ---- stage tile + halo 2: conserved -> (rho, u, v, p) in LDS
         for (int n = threadIdx.x; n < SX * SY; n += NT) {
             int li = n % SX, lj = n / SX;
             int gi = min(max(i0 + li - 2, 0), nx - 1);
             int gj = min(max(j0 + lj - 2, 0), ny - 1);
-            size_t g = (size_t)gj * nx + gi;
+            const unsigned g = (unsigned)(gj * nx + gi);
             const uint8_t c = ctp[g];
             T q[5] = {in0[g], in1[g], in2[g], in3[g], in4[g]};
             if (MODE == 1 && c == CT_GHOST) {
@@ -398,11 +395,11 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
                 const bool own = li >= 2 && li < TX + 2 && lj >= 2 && lj < TY + 2 && i0 + li - 2 < nx && j0 + lj - 2 < ny;
                 if (own) {
                     if (!visc) {
-                        for (int f = 0; f < 5; f++) ((T*)cd.b.U[r.in][f])[g] = q[f];
+                        for (int f = 0; f < 5; f++) Plane<T>(cd.b.U[r.in][f]).put(g, q[f]);
                     } else {
                         for (int f = 0; f < 5; f++) {
-                            T bl = ((const T*)cd.b.U[r.blend][f])[g];
-                            ((T*)cd.b.U[r.out][f])[g] = T(0.5) * bl + T(0.5) * q[f];
+                            T bl = Plane<T>(cd.b.U[r.blend][f])[g];
+                            Plane<T>(cd.b.U[r.out][f]).put(g, T(0.5) * bl + T(0.5) * q[f]);
                         }
                     }
                 }
@@ -488,7 +485,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
             fyt[q] = s_f[5][a + TX];
             fyb[q] = s_f[5][a];
             if (gi >= nx || gj >= ny) continue;
-            const size_t g = (size_t)gj * nx + gi;
+            const unsigned g = (unsigned)(gj * nx + gi);
             const int sc = (lj + 2) * SX + (li + 2);
             const uint8_t c = s_ct[sc];
             T v0[5] = {s_r[sc], in1[g], in2[g], in3[g], in4[g]};
@@ -534,7 +531,6 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
                 nv[0] = nr; nv[1] = nru; nv[2] = nrv; nv[3] = e + ke; nv[4] = e;
             }
             T out[5];
-            float* kc[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
             T carry[5] = {T(0), T(0), T(0), T(0), T(0)};
             if (MODE == 0) {
 #pragma unroll
@@ -542,11 +538,10 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
             } else {
 #pragma unroll
                 for (int f = 0; f < 5; f++) {
-                    T bl = ((const T*)cd.b.U[r.blend][f])[g];
+                    T bl = Plane<T>(cd.b.U[r.blend][f])[g];
                     if (KAHAN) {
-                        kc[f] = cd.b.K[f];
                         T delta = T(0.5) * (nv[f] - bl);
-                        T y = delta - (T)kc[f][g];
+                        T y = delta - (T)Plane<float>(cd.b.K[f])[g];
                         T t = bl + y;
                         carry[f] = (t - bl) - y;
                         out[f] = t;
@@ -558,23 +553,23 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
             if (MODE == 1 && c == CT_GHOST) {
                 // stored by the staging loop
                 if (KAHAN)
-                    for (int f = 0; f < 5; f++) kc[f][g] = 0.0f;
+                    for (int f = 0; f < 5; f++) Plane<float>(cd.b.K[f]).put(g, 0.0f);
                 continue;
             }
             const bool apply_bc = (MODE == 0) || !visc;
             if (apply_bc) {
                 if (c == CT_GHOST) {
                     if (KAHAN && MODE == 1)
-                        for (int f = 0; f < 5; f++) kc[f][g] = 0.0f;
+                        for (int f = 0; f < 5; f++) Plane<float>(cd.b.K[f]).put(g, 0.0f);
                     continue;  // filled by k_ghost
                 }
                 bool touched = bc_pointwise(k, gi, gj, c, out);
                 if (MODE == 1) vmax_local = fmaxf(vmax_local, (float)wave_speed(k, out));
                 if (KAHAN && MODE == 1)
-                    for (int f = 0; f < 5; f++) kc[f][g] = touched ? 0.0f : (float)carry[f];
+                    for (int f = 0; f < 5; f++) Plane<float>(cd.b.K[f]).put(g, touched ? 0.0f : (float)carry[f]);
             }
 #pragma unroll
-            for (int f = 0; f < 5; f++) ((T*)cd.b.U[r.out][f])[g] = out[f];
+            for (int f = 0; f < 5; f++) Plane<T>(cd.b.U[r.out][f]).put(g, out[f]);
         }
         if (MODE == 1 && !visc) {
             const float vm[1] = {vmax_local};
@@ -708,11 +703,11 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
         const CaseConsts<T>& k = cd.c;
         const Roles r = k.visc ? rv : ri;
         const CaseBufs& b = cd.b;
-        const size_t idx = (size_t)((const GhostInfo<T>*)b.ghosts)[it.index].idx;
+        const unsigned idx = (unsigned)((const GhostInfo<T>*)b.ghosts)[it.index].idx;
         T o[5];
         ghost_value(cd, r.out, r.pre0, r.pre1, it.index, o);
         if (active)
-            for (int f = 0; f < 5; f++) ((T*)b.U[r.out][f])[idx] = o[f];
+            for (int f = 0; f < 5; f++) Plane<T>(b.U[r.out][f]).put(idx, o[f]);
         if (FINAL) {
             const float s = active ? (float)wave_speed(k, o) : 0.0f;
             float nu = 0.0f;
@@ -760,18 +755,20 @@ __global__ __launch_bounds__(NT) void k_diffuse(const CaseDesc<T>* __restrict__ 
             if (r.zero_slot) { cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u; }
             if (r.count_step) { cd.b.t[0] += (double)dtf; cd.b.steps[0] += 1; }
         }
-        const T* in[5];
-        T* outp[5];
-        for (int f = 0; f < 5; f++) { in[f] = (const T*)cd.b.U[r.in][f]; outp[f] = (T*)cd.b.U[r.out][f]; }
+        const Plane<T> in[5] = {Plane<T>(cd.b.U[r.in][0]), Plane<T>(cd.b.U[r.in][1]), Plane<T>(cd.b.U[r.in][2]),
+                                Plane<T>(cd.b.U[r.in][3]), Plane<T>(cd.b.U[r.in][4])};
+        const Plane<T> outp[5] = {Plane<T>(cd.b.U[r.out][0]), Plane<T>(cd.b.U[r.out][1]), Plane<T>(cd.b.U[r.out][2]),
+                                  Plane<T>(cd.b.U[r.out][3]), Plane<T>(cd.b.U[r.out][4])};
+        const Plane<uint8_t> ctp(cd.b.ct);
 
         for (int n = threadIdx.x; n < SX * SY; n += NT) {
             int li = n % SX, lj = n / SX;
             int gi = min(max(i0 + li - 2, 0), nx - 1);
             int gj = min(max(j0 + lj - 2, 0), ny - 1);
-            size_t g = (size_t)gj * nx + gi;
+            const unsigned g = (unsigned)(gj * nx + gi);
             T u, v, p, tt;
             primitives(k, in[0][g], in[1][g], in[2][g], in[3][g], in[4][g], u, v, p, tt);
-            s_u[n] = u; s_v[n] = v; s_T[n] = tt; s_ct[n] = cd.b.ct[g];
+            s_u[n] = u; s_v[n] = v; s_T[n] = tt; s_ct[n] = ctp[g];
         }
         __syncthreads();
 
@@ -818,7 +815,7 @@ __global__ __launch_bounds__(NT) void k_diffuse(const CaseDesc<T>* __restrict__ 
             int li = n % TX, lj = n / TX;
             int gi = i0 + li, gj = j0 + lj;
             if (gi >= nx || gj >= ny) continue;
-            size_t g = (size_t)gj * nx + gi;
+            const unsigned g = (unsigned)(gj * nx + gi);
             int s = (lj + 2) * SX + (li + 2);
             int t = (lj + 1) * QX + (li + 1);
             uint8_t c = s_ct[s];
@@ -854,7 +851,7 @@ __global__ __launch_bounds__(NT) void k_diffuse(const CaseDesc<T>* __restrict__ 
                 T mu = (c == CT_SOLID) ? T(0) : sutherland(k, Tin);
                 numax = fmaxf(numax, (float)(mu * Num<T>::rcp(v[0] + T(1e-10f))));
             }
-            for (int f = 0; f < 5; f++) outp[f][g] = v[f];
+            for (int f = 0; f < 5; f++) outp[f].put(g, v[f]);
         }
         if (FINAL) {
             const float vm[2] = {smax, numax};
