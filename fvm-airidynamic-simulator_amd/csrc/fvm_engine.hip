@@ -753,7 +753,7 @@ int Engine::rebuild_lists() {
         }
         if (c->p.enable_viscosity) any_visc_ = true;
         for (int g = 0; g < (int)c->ghosts.size(); g++) ghosts.push_back(WorkItem{c->id, g});
-        clist.push_back(WorkItem{c->id, 0});
+        for (int q = 0; q * REDUCE_CHUNK < nt; q++) clist.push_back(WorkItem{c->id, q});
     }
     // heavy tiles first: dispatched before the rest and spread over the XCDs
     // (dispatch slots go round-robin over them); then the XCD-contiguous order

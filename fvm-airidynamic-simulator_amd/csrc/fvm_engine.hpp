@@ -161,7 +161,7 @@ private:
     WorkItem* d_tiles_all_ = nullptr;
     WorkItem* d_tiles_visc_ = nullptr;
     WorkItem* d_ghosts_ = nullptr;
-    WorkItem* d_case_list_ = nullptr;   // active cases, for the per-tile maxima reduction
+    WorkItem* d_case_list_ = nullptr;   // (case, chunk of REDUCE_CHUNK tiles) items of the per-tile maxima reduction
     WorkItem* scr_tiles_ = nullptr;     // one case's lists for the reset-path launches
     WorkItem* scr_ghosts_ = nullptr;
     int* scr_cnt_ = nullptr;

@@ -651,22 +651,35 @@ __device__ __forceinline__ void wave_slot_max(unsigned* target, bool active, int
     }
 }
 
-// Workgroups [gblocks, gridDim.x) of the final ghost launch: reduce each
-// listed case's per-tile maxima (written by the step's last tiled kernel with
-// plain stores) into its CFL slot, one atomic per case and quantity.
+// Workgroups [gblocks, gridDim.x) of the final ghost launch: reduce the
+// per-tile maxima (written by the step's last tiled kernel with plain stores)
+// into each case's CFL slot, one item (case, chunk of REDUCE_CHUNK tiles) per
+// workgroup pass and one atomic per item and quantity.  Each thread issues its
+// loads of a chunk together, so a chunk costs one memory latency.
 template <class T>
 __device__ void reduce_tile_maxima(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ case_list,
-                                   int ncases, int rblock, int nrblocks, Roles ri, Roles rv) {
+                                   int nitems, int rblock, int nrblocks, Roles ri, Roles rv) {
     __shared__ float s_red[2][NT / 64];
-    for (int ci = rblock; ci < ncases; ci += nrblocks) {
-        const CaseDesc<T>& cd = cases[case_list[ci].case_id];
+    constexpr int PER = REDUCE_CHUNK / NT;
+    for (int ci = rblock; ci < nitems; ci += nrblocks) {
+        const WorkItem it = case_list[ci];
+        const CaseDesc<T>& cd = cases[it.case_id];
         const Roles r = cd.c.visc ? rv : ri;
         const int nt = cd.tiles_x * cd.tiles_y;
-        const float* tm = cd.b.tile_max;
+        const Plane<float> tm(cd.b.tile_max);
+        const int t0 = it.index * REDUCE_CHUNK + threadIdx.x;
+        float a0[PER], a1[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = t0 + q * NT;
+            a0[q] = t < nt ? tm[(unsigned)t] : 0.0f;
+            a1[q] = t < nt ? tm[(unsigned)(nt + t)] : 0.0f;
+        }
         float m0 = 0.0f, m1 = 0.0f;
-        for (int q = threadIdx.x; q < nt; q += NT) {
-            m0 = fmaxf(m0, tm[q]);
-            m1 = fmaxf(m1, tm[nt + q]);
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            m0 = fmaxf(m0, a0[q]);
+            m1 = fmaxf(m1, a1[q]);
         }
         for (int off = 32; off > 0; off >>= 1) {
             m0 = fmaxf(m0, __shfl_xor(m0, off, 64));

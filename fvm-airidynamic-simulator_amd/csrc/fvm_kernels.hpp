@@ -21,9 +21,13 @@ struct KernelCall {
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc);
 bool tile_supported(int tx, int ty, bool fp64);
+// Tiles of one item of the per-tile-maxima reduction: case_list holds one
+// item (case, chunk) per REDUCE_CHUNK tiles of every active case.
+constexpr int REDUCE_CHUNK = 1024;
+
 // Ghost fill over a list of ghost cells.  final_: the step's last launch,
 // which also takes the ghost cells' CFL maxima and, in workgroups
-// [gblocks, grid.x), reduces the per-tile maxima of the cases in case_list.
+// [gblocks, grid.x), reduces the per-tile maxima of the items in case_list.
 void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
                   const int* n, Roles ri, Roles rv, int gblocks = 0, const WorkItem* case_list = nullptr,
                   const int* ncases = nullptr);
