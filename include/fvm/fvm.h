@@ -143,7 +143,10 @@ typedef struct fvm_solver fvm_solver;
 int fvm_create(int device, fvm_solver** out);
 /* Replaces the destructor / freeMemory (solver.cu:2877-2880, 2984-3095). */
 int fvm_destroy(fvm_solver* s);
-/* initialize(params): allocate for nx*ny and reset (solver.cu:3102-3111). */
+/* initialize(params): allocate for nx*ny and reset (solver.cu:3102-3111).
+ * FVM_ERR_INVALID_ARGUMENT when one field plane would exceed 4 GiB
+ * (nx*ny*4 bytes in FP32, nx*ny*8 in FP64): the kernels address a plane with
+ * a 32-bit byte offset. */
 int fvm_initialize(fvm_solver* s, const fvm_params* p);
 /* resize(nx, ny): reallocate if the size changed (solver.cu:3119-3129). */
 int fvm_resize(fvm_solver* s, int nx, int ny);

@@ -271,6 +271,19 @@ def test_invalid_params_rejected(solver, field, value):
     solver.initialize(P.euler_bow_shock())   # the handle stays usable
 
 
+def test_plane_over_4gib_rejected(solver):
+    """32768 x 32769 cells: nx*ny fits int32 but one FP32 field plane would
+    exceed 4 GiB, past the kernels' 32-bit byte offsets.  Rejected by the
+    parameter check, before any device allocation."""
+    p = P.euler_bow_shock(nx=32768, ny=32769, radius=6.0, center=(16.0, 16.0))
+    announce("grid 32768x32769: FP32 plane over 4 GiB")
+    with pytest.raises(fvm_amd.FvmError) as e:
+        solver.initialize(p)
+    assert e.value.code == -1
+    assert "4 GiB" in str(e.value)
+    solver.initialize(P.euler_bow_shock())
+
+
 @pytest.mark.parametrize("nx,ny,r", [(3, 3, 0.0), (7, 5, 1.2), (65, 9, 2.5), (130, 67, 9.0)])
 def test_tiny_and_ragged_grids(solver, nx, ny, r):
     """Grids down to the 3x3 minimum and sizes that are not multiples of any
