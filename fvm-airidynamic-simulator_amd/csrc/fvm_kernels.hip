@@ -236,15 +236,28 @@ __global__ __launch_bounds__(NT) void k_bc_inplace(const CaseDesc<T>* __restrict
 // as in the oracle.
 // ---------------------------------------------------------------------------
 #pragma clang fp contract(off)
-__constant__ int c_ox[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
-__constant__ int c_oy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+// Offsets of neighbour codes 0..7 (the 8-neighbourhood in the scan order of
+// solver.cu:2072), packed two bits per code so that a per-lane code is decoded
+// in registers rather than by a table load.
+constexpr int NB_DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+constexpr int NB_DY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+constexpr unsigned pack_nb(const int (&d)[8]) {
+    unsigned v = 0;
+    for (int m = 0; m < 8; m++) v |= (unsigned)(d[m] + 1) << (2 * m);
+    return v;
+}
+constexpr unsigned NB_DX_BITS = pack_nb(NB_DX), NB_DY_BITS = pack_nb(NB_DY);
+__device__ __forceinline__ int nb_dx(int m) { return (int)((NB_DX_BITS >> (2 * m)) & 3u) - 1; }
+__device__ __forceinline__ int nb_dy(int m) { return (int)((NB_DY_BITS >> (2 * m)) & 3u) - 1; }
 
+// Pre-BC value of field f at cell n: set pre0, or the mean of sets pre0 and
+// pre1 when pre1 >= 0.  Both loads are issued unconditionally (pre0 stands in
+// for a missing pre1), so no branch separates them.
 template <class T>
 __device__ __forceinline__ T pre_val(const CaseBufs& b, int pre0, int pre1, unsigned n, int f) {
-    T a = Plane<T>(b.U[pre0][f])[n];
-    if (pre1 < 0) return a;
-    T c = Plane<T>(b.U[pre1][f])[n];
-    return T(0.5) * a + T(0.5) * c;
+    const T a = Plane<T>(b.U[pre0][f])[n];
+    const T c = Plane<T>(b.U[pre1 >= 0 ? pre1 : pre0][f])[n];
+    return pre1 >= 0 ? T(0.5) * a + T(0.5) * c : a;
 }
 
 // Index into the ghost table of cell (i, j), or -1 when it is not a ghost cell.
@@ -254,36 +267,80 @@ __device__ __forceinline__ int ghost_index(const CaseBufs& b, int i, int j) {
     return Plane<int>(b.gmap)[(unsigned)(lj * b.gbox[2] + li)];
 }
 
-template <class T>
+// EAGER (the final ghost launch, where the chain of loads is the kernel's
+// whole cost): the first candidate's fields are loaded with its tag, before the
+// density check.  Otherwise (the fill folded into stage 2's staging loop, where
+// registers are short): the tag and density first, the winner's fields after.
+template <class T, bool EAGER>
 __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, int gidx, T o[5]) {
     const CaseConsts<T>& k = cd.c;
     const CaseBufs& b = cd.b;
-    const GhostInfo<T> gi = ((const GhostInfo<T>*)b.ghosts)[gidx];
+    const gmem<GhostInfo<T>>* gi = (const gmem<GhostInfo<T>>*)b.ghosts + gidx;
     const int nx = k.nx;
-    const int idx = gi.idx;
+    const int idx = gi->idx;
+    const uint32_t order = gi->order;
     const Plane<uint8_t> ct(b.ct);
-    const T nxs = gi.nx, nys = gi.ny;
-    const int cnt = (int)(gi.order >> 24);
-    int best = -1;
-    bool best_ghost = false;
-    for (int q = 0; q < cnt; q++) {
-        int m = (gi.order >> (3 * q)) & 7;
-        unsigned nidx = (unsigned)(idx + c_oy[m] * nx + c_ox[m]);
-        bool g = ct[nidx] == CT_GHOST;
-        T rn = g ? pre_val<T>(b, pre0, pre1, nidx, 0) : Plane<T>(b.U[src][0])[nidx];
-        if (rn >= k.min_rho) {
-            best = (int)nidx;
-            best_ghost = g;
-            break;
+    const T nxs = gi->nx, nys = gi->ny;
+    const int cnt = (int)(order >> 24);
+    // A ghost neighbour is read at its pre-BC value (set pre0, or the mean of
+    // sets pre0 and pre1 when pre1 >= 0), any other neighbour from set src.
+    // EAGER: a candidate's tag and both versions of its four fields are loaded
+    // together and the version picked afterwards, one memory round trip per
+    // candidate; the first listed one, which almost always passes the density
+    // check, is loaded before the check (the cell itself stands in when none is
+    // listed).
+    const int s0 = pre0 >= 0 ? pre0 : src, s1 = pre1 >= 0 ? pre1 : s0;
+    const bool mean = pre0 >= 0 && pre1 >= 0;
+    auto candidate = [&](unsigned nidx, T cand[4]) {
+        const bool g = ct[nidx] == CT_GHOST;
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            const T cur = Plane<T>(b.U[src][f])[nidx];
+            const T a0 = Plane<T>(b.U[s0][f])[nidx];
+            const T a1 = Plane<T>(b.U[s1][f])[nidx];
+            const T pre = mean ? T(0.5) * a0 + T(0.5) * a1 : a0;
+            cand[f] = g ? pre : cur;
         }
+    };
+    auto nb = [&](int q) {
+        const int m = (order >> (3 * q)) & 7;
+        return (unsigned)(idx + nb_dy(m) * nx + nb_dx(m));
+    };
+    T fv[4];
+    bool found = false;
+    if constexpr (EAGER) {
+        candidate(cnt > 0 ? nb(0) : (unsigned)idx, fv);
+        found = cnt > 0 && fv[0] >= k.min_rho;
+        for (int q = 1; q < cnt && !found; q++) {
+            T cand[4];
+            candidate(nb(q), cand);
+            if (cand[0] >= k.min_rho) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) fv[f] = cand[f];
+                found = true;
+            }
+        }
+    } else {
+        unsigned best = 0;
+        bool best_ghost = false;
+        for (int q = 0; q < cnt; q++) {
+            const unsigned nidx = nb(q);
+            const bool g = ct[nidx] == CT_GHOST;
+            const T rn = g ? pre_val<T>(b, s0, pre1, nidx, 0) : Plane<T>(b.U[src][0])[nidx];
+            if (rn >= k.min_rho) {
+                best = nidx;
+                best_ghost = g;
+                found = true;
+                break;
+            }
+        }
+        if (found)
+            for (int f = 0; f < 4; f++) fv[f] = best_ghost ? pre_val<T>(b, s0, pre1, best, f) : Plane<T>(b.U[src][f])[best];
     }
-    if (best < 0) {
+    if (!found) {
         o[0] = k.rho_inf; o[1] = T(0); o[2] = T(0); o[3] = k.e_inf; o[4] = k.e_inf;
         return;
     }
-    unsigned bn = (unsigned)best;
-    T fv[4];
-    for (int f = 0; f < 4; f++) fv[f] = best_ghost ? pre_val<T>(b, pre0, pre1, bn, f) : Plane<T>(b.U[src][f])[bn];
     T rho_f = fv[0];
     T u_f = fv[1] / rho_f, v_f = fv[2] / rho_f;
     T ke_f = T(0.5) * rho_f * (u_f * u_f + v_f * v_f);
@@ -391,7 +448,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
                 // owns the cell stores it for the later kernels: into the stage-1
                 // set (inviscid: the final ghost fill blends it), or blended into
                 // the output (viscous: the second half step reads it).
-                ghost_value(cd, r.in, r.blend, -1, ghost_index(cd.b, gi, gj), q);
+                ghost_value<T, false>(cd, r.in, r.blend, -1, ghost_index(cd.b, gi, gj), q);
                 const bool own = li >= 2 && li < TX + 2 && lj >= 2 && lj < TY + 2 && i0 + li - 2 < nx && j0 + lj - 2 < ny;
                 if (own) {
                     if (!visc) {
@@ -607,10 +664,10 @@ __global__ __launch_bounds__(NT) void k_ghost_prep(const CaseDesc<T>* __restrict
     int codes[8];
     int cnt = 0;
     for (int m = 0; m < 8; m++) {
-        int ni = i + c_ox[m], nj = j + c_oy[m];
+        int ni = i + nb_dx(m), nj = j + nb_dy(m);
         if (ni < 0 || ni >= nx || nj < 0 || nj >= ny) continue;
         if (!(sdf[(size_t)nj * nx + ni] > 0.0f)) continue;
-        T dxn = (T)c_ox[m], dyn = (T)c_oy[m];
+        T dxn = (T)nb_dx(m), dyn = (T)nb_dy(m);
         T len = ::sqrt(dxn * dxn + dyn * dyn);
         T dot = (dxn * nxs + dyn * nys) / len;
         if (!(dot > T(-1))) continue;
@@ -698,6 +755,34 @@ __device__ void reduce_tile_maxima(const CaseDesc<T>* __restrict__ cases, const 
     }
 }
 
+// One ghost cell per lane: fill, store, and (FINAL) its CFL maxima.
+template <class T, bool FINAL>
+__device__ __forceinline__ void ghost_item(const CaseDesc<T>& cd, const WorkItem it, bool active, Roles ri, Roles rv) {
+    const CaseConsts<T>& k = cd.c;
+    const Roles r = k.visc ? rv : ri;
+    const CaseBufs& b = cd.b;
+    const unsigned idx = (unsigned)((const gmem<GhostInfo<T>>*)b.ghosts)[it.index].idx;
+    // the cell's own pre-BC state (mu of the CFL maxima) is loaded together
+    // with the fill's loads
+    T pv[5];
+    if (FINAL)
+        for (int f = 0; f < 5; f++) pv[f] = pre_val<T>(b, r.pre0, r.pre1, idx, f);
+    T o[5];
+    ghost_value<T, true>(cd, r.out, r.pre0, r.pre1, it.index, o);
+    if (active)
+        for (int f = 0; f < 5; f++) Plane<T>(b.U[r.out][f]).put(idx, o[f]);
+    if (FINAL) {
+        const float s = active ? (float)wave_speed(k, o) : 0.0f;
+        T u_, v_, p_, t_;
+        primitives(k, pv[0], pv[1], pv[2], pv[3], pv[4], u_, v_, p_, t_);
+        const float nu_c = (float)(sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f)));
+        const float nu = (active && k.visc) ? nu_c : 0.0f;
+        // every lane takes part; a lane of another case keeps its own counter
+        wave_slot_max(&b.slots[r.slot_w * 2], active, it.case_id, s);
+        wave_slot_max(&b.slots[r.slot_w * 2 + 1], active && k.visc, it.case_id, nu);
+    }
+}
+
 template <class T, bool FINAL>
 __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ items,
                                               const int* __restrict__ count, Roles ri, Roles rv, int gblocks,
@@ -712,29 +797,13 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
         const int w = base + threadIdx.x;
         const bool active = w < n_items;
         const WorkItem it = items[active ? w : base];
-        const CaseDesc<T>& cd = cases[it.case_id];
-        const CaseConsts<T>& k = cd.c;
-        const Roles r = k.visc ? rv : ri;
-        const CaseBufs& b = cd.b;
-        const unsigned idx = (unsigned)((const GhostInfo<T>*)b.ghosts)[it.index].idx;
-        T o[5];
-        ghost_value(cd, r.out, r.pre0, r.pre1, it.index, o);
-        if (active)
-            for (int f = 0; f < 5; f++) Plane<T>(b.U[r.out][f]).put(idx, o[f]);
-        if (FINAL) {
-            const float s = active ? (float)wave_speed(k, o) : 0.0f;
-            float nu = 0.0f;
-            if (active && k.visc) {
-                T pv[5];
-                for (int f = 0; f < 5; f++) pv[f] = pre_val<T>(b, r.pre0, r.pre1, idx, f);
-                T u_, v_, p_, t_;
-                primitives(k, pv[0], pv[1], pv[2], pv[3], pv[4], u_, v_, p_, t_);
-                nu = (float)(sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f)));
-            }
-            // every lane takes part; a lane of another case keeps its own counter
-            wave_slot_max(&b.slots[r.slot_w * 2], active, it.case_id, s);
-            wave_slot_max(&b.slots[r.slot_w * 2 + 1], active && k.visc, it.case_id, nu);
-        }
+        // A wave whose lanes all hold one case (always so for a single case)
+        // reads the case table through scalar loads with a wave-uniform index;
+        // per-lane indexing would turn every buffer pointer into a vector load
+        // that the field load then waits for.
+        const int c0 = __builtin_amdgcn_readfirstlane(it.case_id);
+        if (__all(it.case_id == c0)) ghost_item<T, FINAL>(cases[c0], it, active, ri, rv);
+        else ghost_item<T, FINAL>(cases[it.case_id], it, active, ri, rv);
     }
 }
 #pragma clang fp contract(on)
