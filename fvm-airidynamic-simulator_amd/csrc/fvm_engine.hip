@@ -768,6 +768,13 @@ int Engine::rebuild_lists() {
     n_tiles_ = (int)oall.size();
     n_visc_ = (int)ovisc.size();
     n_ghosts_ = (int)ghosts.size();
+    // one active case and graphs rebuilt with the lists: the ghost list is the
+    // identity over that case's ghost table, which the final ghost launch can
+    // take without reading it
+    int n_active = 0, last_id = -1;
+    for (HostCase* c : cases_)
+        if (c && c->active && !c->finished) { n_active++; last_id = c->id; }
+    single_case_ = (!persistent_ && n_active == 1) ? last_id : -1;
     if (persistent_) {
         // fixed grids: captured graphs read the lists from device memory and
         // stay valid unless a list buffer moved or the viscous kernels appear
@@ -910,7 +917,7 @@ int Engine::enqueue_step(int ph) {
         rc = timed("diffuse_half2", rc);
     }
     launch_ghost(fp64_, true, dim3(grid_ghosts_ + grid_reduce_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g2,
-                 v_g2, grid_ghosts_, d_case_list_, d_counts_ + 3);
+                 v_g2, grid_ghosts_, d_case_list_, d_counts_ + 3, single_case_);
     rc = timed("ghost2", rc);
     if (rc) return rc;
     if (timing_) {

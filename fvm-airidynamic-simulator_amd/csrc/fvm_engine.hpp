@@ -169,6 +169,7 @@ private:
     int* d_counts_ = nullptr;        // [0] all tiles, [1] viscous tiles, [2] ghosts, [3] active cases
     size_t cap_tiles_ = 0, cap_visc_ = 0, cap_ghosts_ = 0, cap_case_list_ = 0;
     int n_tiles_ = 0, n_visc_ = 0, n_ghosts_ = 0;
+    int single_case_ = -1;              // the one active case of a non-persistent engine, else -1
     int grid_tiles_ = 0, grid_visc_ = 0, grid_ghosts_ = 0, grid_reduce_ = 1;
     bool any_visc_ = false;
     bool persistent_ = false;

@@ -786,7 +786,8 @@ __device__ __forceinline__ void ghost_item(const CaseDesc<T>& cd, const WorkItem
 template <class T, bool FINAL>
 __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ items,
                                               const int* __restrict__ count, Roles ri, Roles rv, int gblocks,
-                                              const WorkItem* __restrict__ case_list, const int* __restrict__ ncases) {
+                                              const WorkItem* __restrict__ case_list, const int* __restrict__ ncases,
+                                              int single_case) {
     if (FINAL && (int)blockIdx.x >= gblocks) {
         reduce_tile_maxima(cases, case_list, *ncases, (int)blockIdx.x - gblocks, (int)gridDim.x - gblocks, ri, rv);
         return;
@@ -796,11 +797,16 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
     for (int base = blockIdx.x * NT; base < n_items; base += gblocks * NT) {
         const int w = base + threadIdx.x;
         const bool active = w < n_items;
+        if (single_case >= 0) {
+            // one case: its list is (case, 0 .. n-1), no item to load
+            ghost_item<T, FINAL>(cases[single_case], WorkItem{single_case, active ? w : base}, active, ri, rv);
+            continue;
+        }
         const WorkItem it = items[active ? w : base];
-        // A wave whose lanes all hold one case (always so for a single case)
-        // reads the case table through scalar loads with a wave-uniform index;
-        // per-lane indexing would turn every buffer pointer into a vector load
-        // that the field load then waits for.
+        // A wave whose lanes all hold one case reads the case table through
+        // scalar loads with a wave-uniform index; per-lane indexing would turn
+        // every buffer pointer into a vector load that the field load then
+        // waits for.
         const int c0 = __builtin_amdgcn_readfirstlane(it.case_id);
         if (__all(it.case_id == c0)) ghost_item<T, FINAL>(cases[c0], it, active, ri, rv);
         else ghost_item<T, FINAL>(cases[it.case_id], it, active, ri, rv);
@@ -1198,14 +1204,15 @@ bool tile_supported(int tx, int ty, bool fp64) {
 }
 
 void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
-                  const int* n, Roles ri, Roles rv, int gblocks, const WorkItem* case_list, const int* ncases) {
+                  const int* n, Roles ri, Roles rv, int gblocks, const WorkItem* case_list, const int* ncases,
+                  int single_case) {
     if (!final_) gblocks = (int)grid.x;
     if (fp64) {
-        if (final_) k_ghost<double, true><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv, gblocks, case_list, ncases);
-        else k_ghost<double, false><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv, gblocks, case_list, ncases);
+        if (final_) k_ghost<double, true><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        else k_ghost<double, false><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv, gblocks, case_list, ncases, single_case);
     } else {
-        if (final_) k_ghost<float, true><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv, gblocks, case_list, ncases);
-        else k_ghost<float, false><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv, gblocks, case_list, ncases);
+        if (final_) k_ghost<float, true><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        else k_ghost<float, false><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv, gblocks, case_list, ncases, single_case);
     }
 }
 
