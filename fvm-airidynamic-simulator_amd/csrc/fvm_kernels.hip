@@ -820,8 +820,21 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
 // ring around the tile.  FINAL: + point-wise BC + CFL maxima (mu from the
 // pre-diffusion temperature as in the reference, rho from the final state).
 // ---------------------------------------------------------------------------
+// Waves per SIMD asked of the register allocator for k_diffuse: as many
+// workgroups as the tile's LDS admits per CU, at most seven, for FP32 (the
+// final half step would otherwise take 79 VGPRs and run six waves where its
+// LDS allows seven; measured ~2% faster with one spilled register); FP64 keeps
+// its registers.
+template <class T, int TX, int TY>
+constexpr int diffuse_waves() {
+    constexpr int SX = TX + 4, SY = TY + 4, QX = TX + 2, QY = TY + 2;
+    constexpr int lds = 3 * SX * SY * (int)sizeof(T) + SX * SY + 5 * QX * QY * (int)sizeof(T) + 32;
+    constexpr int by_lds = 163840 / lds;
+    return sizeof(T) != 4 ? 1 : (by_lds < 7 ? by_lds : 7);
+}
+
 template <class T, int TX, int TY, bool FINAL>
-__global__ __launch_bounds__(NT) void k_diffuse(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
+__global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
                                                 const int* __restrict__ count, Roles r) {
     constexpr int SX = TX + 4, SY = TY + 4, QX = TX + 2, QY = TY + 2, CPT = (TX * TY + NT - 1) / NT;
     __shared__ T s_u[SY * SX], s_v[SY * SX], s_T[SY * SX];
