@@ -106,7 +106,7 @@ def cpu_rate(p, nsteps, threads):
 
 
 def other_configs(with_cpu):
-    """The other baseline rows: the 256x128 Mach-3 Euler case (launch-latency
+    """The other workloads: the 256x128 Mach-3 Euler case (launch-latency
     bound on the GPU) in full, 1000 steps, on the GPU and on the CPU oracle;
     and a 32-case x 200-step subset of the batch mix on the CPU oracle."""
     import fvm_amd
