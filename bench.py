@@ -230,7 +230,7 @@ def served(rate, seconds, step_scale, seed=5, time_limit=None):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus {a.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import fvm_amd

@@ -63,7 +63,7 @@ def read(path: str) -> Snapshot:
     if bytes(raw[:8]) != MAGIC or h.version != VERSION or h.header_bytes != ctypes.sizeof(SnapshotHeader):
         raise ValueError(f"{path}: not an fvm snapshot of version {VERSION}")
     if h.params_bytes != ctypes.sizeof(FvmParams):
-        raise ValueError(f"{path}: parameter block of {h.params_bytes} bytes, this build has {ctypes.sizeof(FvmParams)}")
+        raise ValueError(f"{path}: parameter block of {h.params_bytes} bytes, this library has {ctypes.sizeof(FvmParams)}")
     off = h.header_bytes
     p = FvmParams.from_buffer_copy(raw[off:off + h.params_bytes])
     off += h.params_bytes
