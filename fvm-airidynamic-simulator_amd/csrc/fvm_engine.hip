@@ -797,12 +797,18 @@ int Engine::rebuild_lists() {
 }
 
 void Engine::drop_graphs() {
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
         if (graph_[i]) {
             (void)hipGraphExecDestroy(graph_[i]);
             graph_[i] = nullptr;
         }
+        if (graph_long_[i]) {
+            (void)hipGraphExecDestroy(graph_long_[i]);
+            graph_long_[i] = nullptr;
+        }
+    }
     graph_valid_ = false;
+    long_valid_ = false;
 }
 
 int Engine::flush_timed_set(int set) {
@@ -929,24 +935,40 @@ int Engine::enqueue_step(int ph) {
     return FVM_OK;
 }
 
+// nsteps steps starting at parity first_phase, captured and instantiated.
+int Engine::capture_sequence(int first_phase, int nsteps, hipGraphExec_t* out) {
+    hipGraph_t graph = nullptr;
+    FVM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    int rc = FVM_OK;
+    for (int s = 0; s < nsteps && !rc; s++) rc = enqueue_step(first_phase ^ (s & 1));
+    hipError_t e = hipStreamEndCapture(stream_, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    return FVM_OK;
+}
+
 int Engine::capture_graphs() {
     drop_graphs();
     for (int g = 0; g < 2; g++) {
-        hipGraph_t graph = nullptr;
-        FVM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(g);
-        if (!rc) rc = enqueue_step(1 - g);
-        hipError_t e = hipStreamEndCapture(stream_, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
-        }
-        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&graph_[g], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        int rc = capture_sequence(g, 2, &graph_[g]);
+        if (rc) return rc;
     }
     graph_valid_ = true;
+    return FVM_OK;
+}
+
+int Engine::capture_long_graphs() {
+    for (int g = 0; g < 2; g++) {
+        int rc = capture_sequence(g, LONG_STEPS, &graph_long_[g]);
+        if (rc) return rc;
+    }
+    long_valid_ = true;
     return FVM_OK;
 }
 
@@ -979,6 +1001,14 @@ int Engine::step_graph(int nsteps) {
     if (nsteps >= 2 && !graph_valid_) {
         int rc = capture_graphs();
         if (rc) return rc;
+    }
+    if (nsteps >= LONG_STEPS && !long_valid_) {
+        int rc = capture_long_graphs();
+        if (rc) return rc;
+    }
+    while (nsteps - done >= LONG_STEPS) {
+        FVM_HIP(hipGraphLaunch(graph_long_[phase_], stream_));
+        done += LONG_STEPS;
     }
     while (nsteps - done >= 2) {
         FVM_HIP(hipGraphLaunch(graph_[phase_], stream_));

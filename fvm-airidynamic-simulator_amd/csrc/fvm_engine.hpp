@@ -140,6 +140,8 @@ private:
     int rebuild_lists();
     int enqueue_step(int phase);
     int capture_graphs();
+    int capture_long_graphs();
+    int capture_sequence(int first_phase, int nsteps, hipGraphExec_t* out);
     void drop_graphs();
     void* dev_alloc(size_t bytes);
     void dev_free(void* p);
@@ -177,8 +179,15 @@ private:
 
     int phase_ = 0;                  // step parity: inviscid current set and dt slot
     bool lists_dirty_ = true;
+    // Captured step sequences per starting parity: two steps, and LONG_STEPS
+    // steps (captured on first use), both even so the parity returns to its start.
+    // Consecutive graph launches leave a gap of ~8 us on the device, so long
+    // runs take the long graph.
+    static constexpr int LONG_STEPS = 16;
     hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_long_[2] = {nullptr, nullptr};
     bool graph_valid_ = false;
+    bool long_valid_ = false;
 
     Arena arena_;
     bool use_arena_ = false;
