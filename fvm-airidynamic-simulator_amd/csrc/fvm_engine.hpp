@@ -92,7 +92,7 @@ public:
 
     // stepping
     int step_direct(int nsteps);        // launches without a graph
-    int step_graph(int nsteps);         // replays a captured 2-step graph
+    int step_graph(int nsteps);         // replays captured 16- and 2-step graphs
     int advance(int nsteps, bool use_graph);
     int sync();
 
