@@ -221,13 +221,6 @@ __global__ __launch_bounds__(NT) void k_bc_inplace(const CaseDesc<T>* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Fused RK stage.  MODE 0: stage 1 (update + point-wise BC).  MODE 1: stage 2
-// of an inviscid case (update + blend + point-wise BC + CFL maxima) or of a
-// viscous case (update + blend only, no BC: the second half step follows).
-// KAHAN: the final blend is written as U^n + delta with a carried compensation.
-// ri / rv: roles for inviscid / viscous cases.
-// ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
 // Immersed-boundary ghost value (applyBoundaryConditionsKernel ghost branch,
 // src/solver.cu:2045-2225).  A neighbour that is itself a ghost cell is read
 // at its pre-BC value (set pre0, or the SSP blend of pre0 and pre1); any other
@@ -326,7 +319,7 @@ __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, 
         for (int q = 0; q < cnt; q++) {
             const unsigned nidx = nb(q);
             const bool g = ct[nidx] == CT_GHOST;
-            const T rn = g ? pre_val<T>(b, s0, pre1, nidx, 0) : Plane<T>(b.U[src][0])[nidx];
+            const T rn = g ? pre_val<T>(b, s0, mean ? pre1 : -1, nidx, 0) : Plane<T>(b.U[src][0])[nidx];
             if (rn >= k.min_rho) {
                 best = nidx;
                 best_ghost = g;
@@ -335,7 +328,8 @@ __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, 
             }
         }
         if (found)
-            for (int f = 0; f < 4; f++) fv[f] = best_ghost ? pre_val<T>(b, s0, pre1, best, f) : Plane<T>(b.U[src][f])[best];
+            for (int f = 0; f < 4; f++)
+                fv[f] = best_ghost ? pre_val<T>(b, s0, mean ? pre1 : -1, best, f) : Plane<T>(b.U[src][f])[best];
     }
     if (!found) {
         o[0] = k.rho_inf; o[1] = T(0); o[2] = T(0); o[3] = k.e_inf; o[4] = k.e_inf;
@@ -384,6 +378,13 @@ __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, 
 }
 #pragma clang fp contract(on)
 
+// ---------------------------------------------------------------------------
+// Fused RK stage.  MODE 0: stage 1 (update + point-wise BC).  MODE 1: stage 2
+// of an inviscid case (update + blend + point-wise BC + CFL maxima) or of a
+// viscous case (update + blend only, no BC: the second half step follows).
+// KAHAN: the final blend is written as U^n + delta with a carried compensation.
+// ri / rv: roles for inviscid / viscous cases.
+// ---------------------------------------------------------------------------
 // Waves per SIMD asked of the register allocator for k_stage: six for FP32
 // (at most 80 VGPRs; measured faster than the five an unconstrained allocation
 // allows) unless the tile's LDS already admits fewer workgroups per CU (160
