@@ -240,6 +240,12 @@ constexpr unsigned pack_nb(const int (&d)[8]) {
     return v;
 }
 constexpr unsigned NB_DX_BITS = pack_nb(NB_DX), NB_DY_BITS = pack_nb(NB_DY);
+constexpr bool nb_roundtrip(unsigned bits, const int (&d)[8]) {
+    for (int m = 0; m < 8; m++)
+        if ((int)((bits >> (2 * m)) & 3u) - 1 != d[m]) return false;
+    return true;
+}
+static_assert(nb_roundtrip(NB_DX_BITS, NB_DX) && nb_roundtrip(NB_DY_BITS, NB_DY), "neighbour offset packing");
 __device__ __forceinline__ int nb_dx(int m) { return (int)((NB_DX_BITS >> (2 * m)) & 3u) - 1; }
 __device__ __forceinline__ int nb_dy(int m) { return (int)((NB_DY_BITS >> (2 * m)) & 3u) - 1; }
 
