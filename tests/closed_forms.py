@@ -1,0 +1,285 @@
+"""Cases whose answers are fixed by conservation, characteristics and mirror
+symmetry, shared by the oracle tests and the GPU tests.
+
+Each builder returns (params, state).  With f32=True (the default) the state
+is rounded to float32, the precision of the C ABI's set_state; the oracle's
+float64 tests take it unrounded, so that the uniform parts equal the freestream
+the boundary pass computes from the parameters.  Each check takes what an
+implementation produced and returns the worst deviation from the closed form,
+relative to the field's scale.
+
+- Conservation (updateKernel, solver.cu:1704-1876; fusedViscousDiffusionKernel
+  2251-2511): with the flow uniform near every domain edge, the flux
+  differences and the central viscous divergences telescope, so the domain
+  totals of rho, rho_u, rho_v and E stay fixed to rounding.
+- Characteristic outflow (applyBoundaryConditionsKernel 1891-2030, Riemann
+  invariants 680-841): edge cells get no normal flux difference, and where the
+  normal velocity is supersonic outwards every invariant is taken from the
+  cell itself, so a state that is uniform along the edge stays unchanged.
+  Subsonic outflow takes the incoming invariant from the freestream and
+  changes it.
+- Flat wall (ghost fill 2045-2225): on the flat sides of an unrotated capsule
+  the SDF normal is exactly +-y, so a ghost cell takes the reflection of the
+  cell directly outside it (the state before the boundary pass, also when that
+  cell is itself a ghost), or the freestream fallback when that cell is inside
+  the body too.
+- Sutherland's law (2317-2319) through the CFL dt (3206-3278).
+- A non-finite cell (updateKernel's fallbacks, 1812-1821): NaN reaches no
+  other cell, and nothing outside its domain of dependence changes.
+"""
+import math
+
+import numpy as np
+
+from fvm_amd import params as P
+
+
+def grid(p):
+    x = (np.arange(p.nx) + 0.5) * p.dx
+    y = (np.arange(p.ny) + 0.5) * p.dy
+    return np.meshgrid(x, y)
+
+
+def conservative(p, rho, u, v, pr, f32=True):
+    e = pr / (p.gamma - 1.0)
+    s = np.stack([rho, rho * u, rho * v, e + 0.5 * rho * (u * u + v * v), e])
+    return s.astype(np.float32).astype(np.float64) if f32 else s
+
+
+def no_obstacle(p):
+    p.obstacle_x = p.obstacle_y = -1000.0
+    p.noise_field = P.NOISE_NONE
+    return p
+
+
+# ---------------------------------------------------------------- conservation
+
+def conservation_case(viscous, f32=True):
+    """A density / pressure / velocity bump in Mach-0.5 flow, no obstacle,
+    96x64; the bump stays far from every edge over 20 steps of dt 0.1."""
+    if viscous:
+        p = P.viscous_cylinder(nx=96, ny=64, diameter=8.0, reynolds=20.0, mach=0.5)
+    else:
+        p = P.euler_bow_shock(nx=96, ny=64, mach=0.5, limiter=P.LIMITER_VAN_LEER)
+    no_obstacle(p)
+    p.dt = 0.1
+    X, Y = grid(p)
+    g = np.exp(-((X - 40.0) ** 2 + (Y - 32.0) ** 2) / (2.0 * 3.0 ** 2))
+    rho = 1.0 + 0.2 * g
+    u = p.u_inf + 0.1 * g
+    v = 0.05 * g * (Y - 32.0) / 3.0
+    pr = p.p_inf * (1.0 + 0.3 * g)
+    return p, conservative(p, rho, u, v, pr, f32)
+
+
+CONSERVATION_STEPS = 20
+
+
+def conservation_drift(s0, s1):
+    """max over rho, rho_u, rho_v, E of |sum(s1) - sum(s0)| / sum(|s0|)."""
+    return max(abs(s1[c].sum() - s0[c].sum()) / np.abs(s0[c]).sum() for c in range(4))
+
+
+def bump_moved(s0, s1):
+    """How far the bump evolved (so the case is not trivially static)."""
+    return float(np.abs(s1[0] - s0[0]).max())
+
+
+# ---------------------------------------------------------------- outflow
+
+OUTFLOW_SIDES = ("right", "top", "bottom")
+
+
+def outflow_case(side, supersonic, f32=True):
+    """A state that varies only across `side` and moves out through it at
+    normal Mach ~3 (supersonic) or ~0.5 (subsonic); 64x48, no obstacle."""
+    speed = 3.0 if supersonic else 0.5
+    p = P.euler_bow_shock(nx=64, ny=48, mach=speed if side == "right" else 0.5)
+    no_obstacle(p)
+    p.dt = 0.05
+    X, Y = grid(p)
+    s = X if side == "right" else Y
+    rho = 1.0 + 0.05 * np.sin(0.37 * s)
+    a = speed + 0.1 * np.cos(0.23 * s)
+    pr = p.p_inf * (1.0 + 0.05 * np.cos(0.5 * s))
+    if side == "right":
+        u, v = a, np.zeros_like(a)
+    else:
+        u, v = np.full_like(a, 0.2), (a if side == "top" else -a)
+    return p, conservative(p, rho, u, v, pr, f32)
+
+
+def outflow_edge(p, side, margin=8):
+    """Index of the edge cells checked: away from the corners the other edges
+    influence within one step."""
+    if side == "right":
+        return (slice(None), slice(margin, p.ny - margin), p.nx - 1)
+    j = p.ny - 1 if side == "top" else 0
+    return (slice(None), j, slice(margin, p.nx - margin))
+
+
+def outflow_change(p, side, s0, s1):
+    """Largest relative change of the edge cells over the step: rho, rho_u,
+    rho_v, E against the start; rho_e against E - kinetic energy (the update
+    re-derives it from E)."""
+    idx = outflow_edge(p, side)
+    a, b = s0[idx], s1[idx]
+    worst = 0.0
+    for c in range(4):
+        scale = max(np.abs(a[c]).max(), 1.0 if c in (1, 2) else 0.0)
+        worst = max(worst, np.abs(b[c] - a[c]).max() / scale)
+    ke = 0.5 * (a[1] ** 2 + a[2] ** 2) / a[0]
+    worst = max(worst, np.abs(b[4] - (a[3] - ke)).max() / np.abs(a[3] - ke).max())
+    return worst
+
+
+# ---------------------------------------------------------------- flat wall
+
+FLAT_WALL_MODES = ("noslip", "slip", "isothermal")
+CAPSULE = dict(cx=48.0, cy=32.0, r=17.8)   # flat sides at y = 32 +- 8.9, x in [30.2, 65.8]
+FLAT_COLUMNS = range(33, 63)               # both neighbours in x on the flat part too
+
+
+def flat_wall_case(mode, f32=True):
+    """Unrotated capsule on a non-uniform state, 96x64.  Its flat sides put the
+    ghost rows at SDF -1.4, -0.4, +0.6 (band 1.5)."""
+    if mode == "slip":
+        p = P.euler_bow_shock(nx=96, ny=64, mach=0.5)
+        p.wall_mode = P.WALL_SLIP
+    else:
+        p = P.viscous_cylinder(nx=96, ny=64, diameter=8.0, reynolds=50.0, mach=0.5)
+        if mode == "isothermal":
+            p.adiabatic_wall = 0
+            p.T_wall = 1.6
+    p.noise_field = P.NOISE_NONE
+    p.obstacle_shape = P.SHAPE_CAPSULE
+    p.obstacle_rotation = 0.0
+    p.obstacle_x, p.obstacle_y, p.obstacle_r = CAPSULE["cx"], CAPSULE["cy"], CAPSULE["r"]
+    X, Y = grid(p)
+    rho = 1.0 + 0.1 * np.sin(0.3 * X + 0.2 * Y)
+    u = 0.5 + 0.1 * np.cos(0.25 * Y + 0.1 * X)
+    v = 0.08 * np.sin(0.4 * X - 0.3 * Y)
+    pr = p.p_inf * (1.0 + 0.05 * np.cos(0.1 * X + 0.3 * Y))
+    return p, conservative(p, rho, u, v, pr, f32)
+
+
+def flat_sdf(j):
+    """Exact signed distance of row j's centres to the capsule's flat sides."""
+    cap = np.float32(CAPSULE["r"]) * np.float32(0.5)
+    return abs(j + 0.5 - CAPSULE["cy"]) - float(cap)
+
+
+def mirror(p, mode, f):
+    """Reflection of cell state f across a wall whose normal is +-y."""
+    gm1, R = p.gamma - 1.0, p.r_gas
+    rho, u, v = f[0], f[1] / f[0], f[2] / f[0]
+    pr = gm1 * (f[3] - 0.5 * rho * (u * u + v * v))
+    if mode == "slip":
+        ug, vg, rg = u, -v, rho
+    else:
+        ug, vg = -u, -v
+        if mode == "isothermal":
+            rg = pr / (R * (2.0 * p.T_wall - pr / (rho * R)))
+        else:
+            rg = rho
+    e = pr / gm1
+    return np.array([rg, rg * ug, rg * vg, e + 0.5 * rg * (ug * ug + vg * vg), e])
+
+
+def flat_wall_expected_rows():
+    """Ghost rows next to the flat sides: (row, row of the cell it reflects or
+    None for the freestream fallback)."""
+    out = []
+    for j in range(64):
+        d = flat_sdf(j)
+        if -1.5 <= d < 1.5:
+            nj = j + 1 if j + 0.5 > CAPSULE["cy"] else j - 1
+            out.append((j, nj if d + 1.0 > 0.0 else None))
+    return out
+
+
+def flat_wall_error(p, mode, s0, ct, s1):
+    """Worst relative deviation of the ghost cells on the flat sides from the
+    reflection of the state before the boundary pass; also checks that those
+    rows are the ghost rows and that fluid cells were left alone."""
+    rows = flat_wall_expected_rows()
+    assert [j for j, _ in rows] == [22, 23, 24, 39, 40, 41]
+    e_inf = p.p_inf / (p.gamma - 1.0)
+    fallback = np.array([p.rho_inf, 0.0, 0.0, e_inf, e_inf])
+    worst = 0.0
+    for i in FLAT_COLUMNS:
+        col_ghost = [j for j in range(p.ny) if ct[j, i] == P.CELL_GHOST]
+        assert col_ghost == [j for j, _ in rows], (i, col_ghost)
+        for j, nj in rows:
+            want = fallback if nj is None else mirror(p, mode, s0[:, nj, i])
+            got = s1[:, j, i]
+            scale = np.array([1.0, 1.0, 1.0, abs(want[3]), abs(want[4])])
+            worst = max(worst, float((np.abs(got - want) / scale).max()))
+    fluid = ct == P.CELL_FLUID
+    fluid[:, 0] = fluid[:, -1] = False
+    fluid[0, :] = fluid[-1, :] = False
+    worst_fluid = float(np.abs(s1[:, fluid] - s0[:, fluid]).max())
+    return worst, worst_fluid
+
+
+# ---------------------------------------------------------------- Sutherland
+
+SUTHERLAND_T = (1.0, 4.0)   # multiples of T_ref
+
+
+def sutherland_case(t_mult):
+    """Uniform freestream at T = t_mult * T_ref with mu_ref = 2, so the viscous
+    limit sets the CFL dt; 32x32, no obstacle."""
+    p = P.viscous_cylinder(nx=32, ny=32, diameter=8.0, reynolds=20.0, mach=0.3)
+    p.mu_ref = 2.0
+    p.T_inf = float(np.float32(t_mult * p.t_ref))
+    P.compute_derived(p)
+    no_obstacle(p)
+    return p
+
+
+def sutherland_dt(p):
+    """dt = min(cfl dx / (|u| + c), cfl_visc dx^2 / nu), nu = mu(T) / rho with
+    Sutherland's law mu = mu_ref (T/T_ref)^(3/2) (T_ref + S)/(T + S)."""
+    T, tr, s = p.T_inf, p.t_ref, p.s_sutherland
+    mu = p.mu_ref * (T / tr) ** 1.5 * (tr + s) / (T + s)
+    nu = mu / p.rho_inf
+    c = math.sqrt(p.gamma * p.r_gas * T)
+    dx = min(p.dx, p.dy)
+    return min(p.cfl * dx / (abs(p.u_inf) + c), p.cfl_visc * dx * dx / nu), nu
+
+
+# ---------------------------------------------------------------- non-finite cell
+
+NAN_CELL = (24, 30)
+NAN_STEPS = 3
+
+
+def nonfinite_case(f32=True):
+    """Mach-0.5 freestream, no obstacle, 64x48, with rho_u of one cell NaN."""
+    p = P.euler_bow_shock(nx=64, ny=48, mach=0.5)
+    no_obstacle(p)
+    p.dt = 0.1
+    rho = np.full((p.ny, p.nx), p.rho_inf)
+    s = conservative(p, rho, np.full_like(rho, p.u_inf), np.zeros_like(rho), np.full_like(rho, p.p_inf), f32)
+    s[1][NAN_CELL] = np.nan
+    return p, s
+
+
+def nonfinite_spread(s0, s1, nsteps):
+    """(cells other than the injected one holding a non-finite value, largest
+    relative change of a cell outside the injected cell's domain of dependence).
+
+    Every field's update falls back to its old value when the new one is not
+    finite, so NaN reaches no other cell.  The cells next to it do change:
+    the Riemann solvers' min/max wave speeds drop a NaN operand, so some of
+    their faces get finite fluxes.  Per RK stage a cell sees faces up to two
+    cells away along x and y, so after n steps nothing beyond Chebyshev
+    distance 4n from the injected cell may differ from the freestream."""
+    bad = ~np.isfinite(s1).all(axis=0)
+    bad[NAN_CELL] = False
+    jj, ii = np.indices(s1.shape[1:])
+    far = np.maximum(np.abs(jj - NAN_CELL[0]), np.abs(ii - NAN_CELL[1])) > 4 * nsteps
+    ref = s0[:, far]
+    scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)
+    return int(bad.sum()), float((np.abs(s1[:, far] - ref) / scale).max())

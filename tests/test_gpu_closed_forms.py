@@ -1,0 +1,106 @@
+"""The HIP path, through the C ABI, against the closed forms of closed_forms.py:
+conservation, characteristic outflow, flat-wall reflection, Sutherland's law
+and the non-finite fallback.  The same checks pin the oracle
+(test_oracle_closed_forms.py); here they hold for the GPU's FP32, Kahan and
+FP64 arithmetic, with bars set by each one's rounding."""
+import numpy as np
+import pytest
+
+import closed_forms as C
+import fvm_amd
+from fvm_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = {"fp32": P.PRECISION_FP32, "kahan": P.PRECISION_FP32_KAHAN, "fp64": P.PRECISION_FP64}
+
+
+def announce(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = fvm_amd.CFDSolver(0)
+    yield s
+    s.close()
+
+
+def start(solver, p, s0, precision):
+    p.precision = PRECISIONS[precision]
+    solver.initialize(p)
+    solver.set_state(s0)
+
+
+@pytest.mark.parametrize("viscous,precision", [(False, "fp32"), (False, "kahan"), (False, "fp64"),
+                                               (True, "fp32"), (True, "fp64")])   # Kahan is inviscid only
+def test_conservation_telescopes(solver, viscous, precision):
+    # The state is rounded to float32, so the freestream it holds differs from
+    # the one the boundary pass derives from the parameters by ~1e-8: the edge
+    # cells move by that much, which bounds the FP64 path's drift.
+    p, s0 = C.conservation_case(viscous)
+    announce(f"conservation viscous={viscous} {precision} 96x64 {C.CONSERVATION_STEPS} steps")
+    start(solver, p, s0, precision)
+    solver.advance(p.copy(), C.CONSERVATION_STEPS)
+    s1 = solver.state64()
+    drift = C.conservation_drift(s0, s1)
+    print(f"  drift {drift:.1e}", flush=True)
+    assert C.bump_moved(s0, s1) > 1e-3
+    assert drift < (1e-8 if precision == "fp64" else 1e-6)
+    sums = solver.conservation_sums()
+    assert np.allclose(sums, s1[:4].sum(axis=(1, 2)), rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("side", C.OUTFLOW_SIDES)
+def test_supersonic_outflow_edge_unchanged(solver, side, precision):
+    for supersonic in (True, False):
+        p, s0 = C.outflow_case(side, supersonic)
+        announce(f"outflow {side} supersonic={supersonic} {precision} 64x48 1 step")
+        start(solver, p, s0, precision)
+        solver.advance(p.copy(), 1)
+        change = C.outflow_change(p, side, s0, solver.state64())
+        print(f"  edge change {change:.1e}", flush=True)
+        if supersonic:
+            # FP32 takes p / rho^gamma through exp and log
+            assert change < (1e-12 if precision == "fp64" else 2e-6), change
+        else:
+            assert change > 1e-3, change
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("mode", C.FLAT_WALL_MODES)
+def test_flat_wall_ghost_reflection(solver, mode, precision):
+    p, s0 = C.flat_wall_case(mode)
+    announce(f"flat wall {mode} {precision} 96x64 one boundary pass")
+    start(solver, p, s0, precision)
+    solver.updateObstacleGeometry(p)     # same geometry: one boundary pass
+    err, fluid = C.flat_wall_error(p, mode, s0, solver.getCellTypes(), solver.state64())
+    print(f"  ghost error {err:.1e}", flush=True)
+    assert err < (1e-9 if precision == "fp64" else 1e-6), err
+    assert fluid == 0.0
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("t_mult", C.SUTHERLAND_T)
+def test_sutherland_through_cfl_dt(solver, t_mult, precision):
+    p = C.sutherland_case(t_mult)
+    p.precision = PRECISIONS[precision]
+    announce(f"Sutherland T={t_mult} T_ref {precision} 32x32 CFL dt")
+    solver.initialize(p)
+    dt, _ = C.sutherland_dt(p)
+    assert solver.computeStableTimeStep(p) == pytest.approx(dt, rel=2e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_nonfinite_cell_stays_confined(solver, precision):
+    p, s0 = C.nonfinite_case()
+    announce(f"NaN in one cell {precision} 64x48 {C.NAN_STEPS} steps")
+    start(solver, p, s0, precision)
+    solver.advance(p.copy(), C.NAN_STEPS)
+    s1 = solver.state64()
+    spread, dev = C.nonfinite_spread(s0, s1, C.NAN_STEPS)
+    print(f"  other non-finite cells {spread}, far change {dev:.1e}", flush=True)
+    assert spread == 0
+    assert dev < (1e-7 if precision == "fp64" else 1e-6), dev
+    assert np.abs(s1[0] - s0[0]).max() > 1e-2

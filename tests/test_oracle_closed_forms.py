@@ -1,0 +1,90 @@
+"""The oracle's update, boundary pass and viscosity against closed forms:
+conservation, characteristic outflow, flat-wall reflection, Sutherland's law
+and the non-finite fallback (cases in closed_forms.py)."""
+import numpy as np
+import pytest
+
+import closed_forms as C
+import oracle as O
+
+
+@pytest.mark.parametrize("viscous", [False, True])
+def test_conservation_telescopes(viscous):
+    p, s0 = C.conservation_case(viscous, f32=False)
+    o = O.Oracle(p, fp64=True)
+    o.set_state(s0)
+    o.step(C.CONSERVATION_STEPS, p.copy())
+    s1 = o.state()
+    assert C.bump_moved(s0, s1) > 1e-3
+    assert C.conservation_drift(s0, s1) < 1e-13
+    sums = o.conservation_sums()
+    assert np.allclose(sums, s1[:4].sum(axis=(1, 2)), rtol=1e-13, atol=1e-12)
+    o.close()
+
+
+@pytest.mark.parametrize("side", C.OUTFLOW_SIDES)
+def test_supersonic_outflow_edge_unchanged(side):
+    for supersonic in (True, False):
+        p, s0 = C.outflow_case(side, supersonic, f32=False)
+        o = O.Oracle(p, fp64=True)
+        o.set_state(s0)
+        o.step(1, p.copy())
+        change = C.outflow_change(p, side, s0, o.state())
+        if supersonic:
+            assert change < 1e-13, change
+        else:
+            assert change > 1e-3, change   # incoming invariant from the freestream
+        o.close()
+
+
+@pytest.mark.parametrize("mode", C.FLAT_WALL_MODES)
+def test_flat_wall_ghost_reflection(mode):
+    p, s0 = C.flat_wall_case(mode, f32=False)
+    o = O.Oracle(p, fp64=True)
+    o.set_state(s0)
+    o.update_obstacle(p)          # same geometry: one boundary pass
+    err, fluid = C.flat_wall_error(p, mode, s0, o.cell_types(), o.state())
+    assert err < 1e-9, err
+    assert fluid == 0.0
+    o.close()
+
+
+def test_flat_wall_catches_a_gauss_seidel_fill():
+    """The row at SDF -0.4 reflects the ghost row above it as it was before
+    the pass; its reflection after the pass differs by far more than the bar."""
+    p, s0 = C.flat_wall_case("noslip", f32=False)
+    o = O.Oracle(p, fp64=True)
+    o.set_state(s0)
+    o.update_obstacle(p)
+    s1 = o.state()
+    i = 40
+    after = C.mirror(p, "noslip", s1[:, 41, i])
+    before = C.mirror(p, "noslip", s0[:, 41, i])
+    assert np.abs(s1[:, 40, i] - before).max() < 1e-9
+    assert np.abs(after - before).max() > 1e-2
+    o.close()
+
+
+@pytest.mark.parametrize("t_mult", C.SUTHERLAND_T)
+def test_sutherland_through_cfl_dt(t_mult):
+    p = C.sutherland_case(t_mult)
+    o = O.Oracle(p, fp64=True)
+    dt, nu = C.sutherland_dt(p)
+    _, mn = o.maxima()
+    assert mn == pytest.approx(nu, rel=1e-6)
+    assert o.stable_dt(p) == pytest.approx(dt, rel=1e-6)
+    assert dt == pytest.approx(p.cfl_visc / nu, rel=1e-12)   # the viscous limit is the one taken
+    o.close()
+
+
+def test_nonfinite_cell_stays_confined():
+    p, s0 = C.nonfinite_case(f32=False)
+    o = O.Oracle(p, fp64=True)
+    o.set_state(s0)
+    o.step(C.NAN_STEPS, p.copy())
+    s1 = o.state()
+    spread, dev = C.nonfinite_spread(s0, s1, C.NAN_STEPS)
+    assert spread == 0
+    assert dev < 1e-13, dev
+    assert np.abs(s1[0] - s0[0]).max() > 1e-2      # the neighbours did see it
+    o.close()
