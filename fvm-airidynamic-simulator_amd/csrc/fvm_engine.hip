@@ -114,12 +114,14 @@ Engine::~Engine() {
     for (auto& ts : tset_)
         for (auto& ev : ts.ev)
             if (ev) (void)hipEventDestroy(ev);
+    if (ev_null_) (void)hipEventDestroy(ev_null_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 int Engine::init_stream() {
     FVM_HIP(hipSetDevice(device_));
     if (!stream_) FVM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (!ev_null_) FVM_HIP(hipEventCreateWithFlags(&ev_null_, hipEventDisableTiming));
     if (!d_counts_) {
         FVM_HIP(hipMalloc((void**)&d_counts_, 8 * sizeof(int)));
         FVM_HIP(hipMemset(d_counts_, 0, 8 * sizeof(int)));
@@ -1156,8 +1158,16 @@ int Engine::field(int id, int fieldno, float* dev_out) {
     HostCase* hc = get(id);
     if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
     if (fieldno < 0 || fieldno > 5) return set_error(FVM_ERR_INVALID_ARGUMENT, "unknown field");
+    // The reference writes these buffers on the legacy default stream, so its
+    // callers order their own default-stream work (graphics interop, copies)
+    // around the call without synchronising.  The solver's stream does not
+    // synchronise with the null stream, so wait here for the caller's earlier
+    // null-stream work and return only once the buffer is written.
+    FVM_HIP(hipEventRecord(ev_null_, nullptr));
+    FVM_HIP(hipStreamWaitEvent(stream_, ev_null_, 0));
     launch_field(fp64_, stream_, d_cases_, id, (size_t)hc->p.nx * hc->p.ny, current_set(*hc), fieldno, dev_out);
     FVM_HIP(hipGetLastError());
+    FVM_HIP(hipStreamSynchronize(stream_));
     return FVM_OK;
 }
 

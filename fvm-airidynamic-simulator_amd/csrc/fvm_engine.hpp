@@ -154,6 +154,7 @@ private:
     bool fp64_, kahan_;
     int tx_ = 64, ty_ = 8;
     hipStream_t stream_ = nullptr;
+    hipEvent_t ev_null_ = nullptr;   // marks the caller's null-stream work before a readback
     std::vector<HostCase*> cases_;
     int max_cases_ = 0;
     void* d_cases_ = nullptr;        // CaseDesc<T>[max_cases_]

@@ -174,7 +174,9 @@ int fvm_get_cell_types(fvm_solver* s, uint8_t* host);
 int fvm_get_sdf(fvm_solver* s, float* host);
 /* convertCellTypesToDevice / compute{Temperature,Pressure,Density,
  * VelocityMag,Mach}ToDevice: nx*ny floats into caller device memory
- * (solver.cu:3444-3459, 3527-3671). */
+ * (solver.cu:3444-3459, 3527-3671).  The write is ordered after the caller's
+ * earlier work on the null stream and is complete when the call returns, as
+ * the reference's default-stream launch is for its callers. */
 int fvm_field_to_device(fvm_solver* s, int field, float* dev);
 /* The same fields into caller host memory. */
 int fvm_field_to_host(fvm_solver* s, int field, float* host);
