@@ -29,7 +29,7 @@ HBM_PEAK = 8.0e12  # bytes/s, MI355X
 # Algorithmic HBM bytes per cell for each kernel of one step (FP32 state of
 # 5 fields = 20 B, cell tag 1 B): what the kernel must read and write once.
 KERNEL_BYTES_PER_CELL = {
-    "diffuse_half1": 20 + 1 + 20,        # read U^n, tag; write U_d
+    "diffuse_half1": 20 + 1 + 16,        # read U^n, tag; write U_d (rho is U^n's plane)
     "stage1": 20 + 1 + 20,               # read U_d (or U^n), tag; write U*
     "stage2": 20 + 20 + 1 + 20,          # read U*, U_d (blend), tag; write blend
     "diffuse_half2": 20 + 1 + 20,        # read blend, tag; write U^{n+1}

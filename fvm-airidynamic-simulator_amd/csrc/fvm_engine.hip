@@ -359,6 +359,11 @@ int Engine::upload_desc(HostCase& hc) {
     for (int s = 0; s < 4; s++)
         for (int f = 0; f < 5; f++)
             b.U[s][f] = (s < hc.nsets) ? (void*)(base + L.sets_off + L.plane * (s * 5 + f)) : nullptr;
+    // The first viscous half step (A -> D) does not change rho, so D's density
+    // plane is A's: that half step writes four fields, and the stage kernels
+    // and the blend read U^n's density in place.  A's density is next written
+    // by the final half step, after the blend, the last reader of D.
+    if (hc.nsets > 3) b.U[3][0] = b.U[0][0];
     if (kahan_)
         for (int f = 0; f < 5; f++) b.K[f] = (float*)(base + L.k_off + round256(L.n * 4) * f);
     b.ct = (uint8_t*)(base + L.ct_off);

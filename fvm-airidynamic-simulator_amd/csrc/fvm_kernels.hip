@@ -959,7 +959,8 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
                 T mu = (c == CT_SOLID) ? T(0) : sutherland(k, Tin);
                 numax = fmaxf(numax, (float)(mu * Num<T>::rcp(v[0] + T(1e-10f))));
             }
-            for (int f = 0; f < 5; f++) outp[f].put(g, v[f]);
+            // the first half step leaves rho alone: set D's density plane is set A's
+            for (int f = FINAL ? 0 : 1; f < 5; f++) outp[f].put(g, v[f]);
         }
         if (FINAL) {
             const float vm[2] = {smax, numax};
