@@ -18,11 +18,12 @@ relative to the field's scale.
   cell itself, so a state that is uniform along the edge stays unchanged.
   Subsonic outflow takes the incoming invariant from the freestream and
   changes it.
-- Flat wall (ghost fill 2045-2225): on the flat sides of an unrotated capsule
+- Flat walls (ghost fill 2045-2225): on the flat sides of an unrotated capsule
   the SDF normal is exactly +-y, so a ghost cell takes the reflection of the
   cell directly outside it (the state before the boundary pass, also when that
   cell is itself a ghost), or the freestream fallback when that cell is inside
-  the body too.
+  the body too.  On the 45-degree sides of a diamond the same holds with the
+  diagonal neighbour and the normal (+-1, +-1)/sqrt(2).
 - Sutherland's law (2317-2319) through the CFL dt (3206-3278).
 - A non-finite cell (updateKernel's fallbacks, 1812-1821): NaN reaches no
   other cell, and nothing outside its domain of dependence changes.
@@ -140,9 +141,9 @@ CAPSULE = dict(cx=48.0, cy=32.0, r=17.8)   # flat sides at y = 32 +- 8.9, x in [
 FLAT_COLUMNS = range(33, 63)               # both neighbours in x on the flat part too
 
 
-def flat_wall_case(mode, f32=True):
-    """Unrotated capsule on a non-uniform state, 96x64.  Its flat sides put the
-    ghost rows at SDF -1.4, -0.4, +0.6 (band 1.5)."""
+def wall_flow(mode, f32=True):
+    """Mach-0.5 flow on a smooth non-uniform state, 96x64, wall mode `mode`;
+    the caller places the obstacle."""
     if mode == "slip":
         p = P.euler_bow_shock(nx=96, ny=64, mach=0.5)
         p.wall_mode = P.WALL_SLIP
@@ -152,9 +153,7 @@ def flat_wall_case(mode, f32=True):
             p.adiabatic_wall = 0
             p.T_wall = 1.6
     p.noise_field = P.NOISE_NONE
-    p.obstacle_shape = P.SHAPE_CAPSULE
     p.obstacle_rotation = 0.0
-    p.obstacle_x, p.obstacle_y, p.obstacle_r = CAPSULE["cx"], CAPSULE["cy"], CAPSULE["r"]
     X, Y = grid(p)
     rho = 1.0 + 0.1 * np.sin(0.3 * X + 0.2 * Y)
     u = 0.5 + 0.1 * np.cos(0.25 * Y + 0.1 * X)
@@ -163,19 +162,32 @@ def flat_wall_case(mode, f32=True):
     return p, conservative(p, rho, u, v, pr, f32)
 
 
+def flat_wall_case(mode, f32=True):
+    """Unrotated capsule on a non-uniform state, 96x64.  Its flat sides put the
+    ghost rows at SDF -1.4, -0.4, +0.6 (band 1.5)."""
+    p, s = wall_flow(mode, f32)
+    p.obstacle_shape = P.SHAPE_CAPSULE
+    p.obstacle_x, p.obstacle_y, p.obstacle_r = CAPSULE["cx"], CAPSULE["cy"], CAPSULE["r"]
+    return p, s
+
+
 def flat_sdf(j):
     """Exact signed distance of row j's centres to the capsule's flat sides."""
     cap = np.float32(CAPSULE["r"]) * np.float32(0.5)
     return abs(j + 0.5 - CAPSULE["cy"]) - float(cap)
 
 
-def mirror(p, mode, f):
-    """Reflection of cell state f across a wall whose normal is +-y."""
+def mirror(p, mode, f, n=(0.0, 1.0)):
+    """Reflection of cell state f across a wall with unit normal n: no-slip
+    walls reverse the velocity, slip walls reverse its normal component
+    (v - 2 (v.n) n); the wall keeps p, and T (adiabatic) or 2 T_wall - T
+    (isothermal)."""
     gm1, R = p.gamma - 1.0, p.r_gas
     rho, u, v = f[0], f[1] / f[0], f[2] / f[0]
     pr = gm1 * (f[3] - 0.5 * rho * (u * u + v * v))
     if mode == "slip":
-        ug, vg, rg = u, -v, rho
+        vn = u * n[0] + v * n[1]
+        ug, vg, rg = u - 2.0 * vn * n[0], v - 2.0 * vn * n[1], rho
     else:
         ug, vg = -u, -v
         if mode == "isothermal":
@@ -220,6 +232,46 @@ def flat_wall_error(p, mode, s0, ct, s1):
     fluid[0, :] = fluid[-1, :] = False
     worst_fluid = float(np.abs(s1[:, fluid] - s0[:, fluid]).max())
     return worst, worst_fluid
+
+
+# ---------------------------------------------------------------- 45-degree wall
+
+DIAMOND = dict(cx=48.3, cy=32.7, r=20.0)
+
+
+def oblique_wall_case(mode, f32=True):
+    """Unrotated diamond |x| + |y| = r on the same flow: its sides are straight
+    at 45 degrees, so away from the vertices the SDF normal is
+    (+-1, +-1)/sqrt(2) and the neighbour along it is a diagonal one."""
+    p, s = wall_flow(mode, f32)
+    p.obstacle_shape = P.SHAPE_DIAMOND
+    p.obstacle_x, p.obstacle_y, p.obstacle_r = DIAMOND["cx"], DIAMOND["cy"], DIAMOND["r"]
+    return p, s
+
+
+def oblique_wall_error(p, mode, s0, ct, sdf, s1):
+    """Worst relative deviation of the ghost cells on the diamond's sides (3
+    cells or more from a vertex, where the SDF is linear over the normal's
+    stencil) from the reflection across the side of the diagonal neighbour's
+    state before the pass, or the freestream fallback when that neighbour is
+    inside the body (SDF <= 0).  Returns (error, cells checked)."""
+    e_inf = p.p_inf / (p.gamma - 1.0)
+    fallback = np.array([p.rho_inf, 0.0, 0.0, e_inf, e_inf])
+    worst, count = 0.0, 0
+    for j, i in zip(*np.nonzero(ct == P.CELL_GHOST)):
+        rx, ry = i + 0.5 - DIAMOND["cx"], j + 0.5 - DIAMOND["cy"]
+        if min(abs(rx), abs(ry)) < 3.0:
+            continue
+        sx, sy = int(np.sign(rx)), int(np.sign(ry))
+        nj, ni = j + sy, i + sx
+        if sdf[nj, ni] > 0.0:
+            want = mirror(p, mode, s0[:, nj, ni], (sx / math.sqrt(2.0), sy / math.sqrt(2.0)))
+        else:
+            want = fallback
+        scale = np.array([1.0, 1.0, 1.0, abs(want[3]), abs(want[4])])
+        worst = max(worst, float((np.abs(s1[:, j, i] - want) / scale).max()))
+        count += 1
+    return worst, count
 
 
 # ---------------------------------------------------------------- Sutherland

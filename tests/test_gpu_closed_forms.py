@@ -82,6 +82,19 @@ def test_flat_wall_ghost_reflection(solver, mode, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("mode", C.FLAT_WALL_MODES)
+def test_oblique_wall_ghost_reflection(solver, mode, precision):
+    p, s0 = C.oblique_wall_case(mode)
+    announce(f"45-degree wall {mode} {precision} 96x64 one boundary pass")
+    start(solver, p, s0, precision)
+    solver.updateObstacleGeometry(p)
+    err, count = C.oblique_wall_error(p, mode, s0, solver.getCellTypes(), solver.getSDF(), solver.state64())
+    print(f"  ghost error {err:.1e} over {count} cells", flush=True)
+    assert count > 100
+    assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("t_mult", C.SUTHERLAND_T)
 def test_sutherland_through_cfl_dt(solver, t_mult, precision):
     p = C.sutherland_case(t_mult)

@@ -49,6 +49,18 @@ def test_flat_wall_ghost_reflection(mode):
     o.close()
 
 
+@pytest.mark.parametrize("mode", C.FLAT_WALL_MODES)
+def test_oblique_wall_ghost_reflection(mode):
+    p, s0 = C.oblique_wall_case(mode, f32=False)
+    o = O.Oracle(p, fp64=True)
+    o.set_state(s0)
+    o.update_obstacle(p)
+    err, count = C.oblique_wall_error(p, mode, s0, o.cell_types(), o.sdf(), o.state())
+    assert count > 100
+    assert err < 2e-6, err     # the normal comes from FP32 SDF differences
+    o.close()
+
+
 def test_flat_wall_catches_a_gauss_seidel_fill():
     """The row at SDF -0.4 reflects the ghost row above it as it was before
     the pass; its reflection after the pass differs by far more than the bar."""
