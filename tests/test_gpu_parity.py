@@ -250,6 +250,40 @@ def test_readback_fields_and_stats(solver):
     assert solver.getSimulationMemoryUsage() > 20 * p.nx * p.ny
 
 
+def test_resize_then_reset():
+    """resize (solver.cu:3119-3129) reallocates for the new grid and is a
+    no-op at the current size or for a non-positive size; the front end then
+    calls reset with matching params (main.cpp:1239), which must give the new
+    grid's initial state and step like a fresh solver."""
+    announce("resize 128x64 -> 200x100, reset, 20 steps")
+    with fvm_amd.CFDSolver(0) as s:
+        with pytest.raises(fvm_amd.FvmError):
+            s.resize(64, 32)                       # before initialize
+        p = P.euler_bow_shock(nx=128, ny=64, radius=10.0, center=(40.0, 32.0))
+        p.dt = 0.1
+        s.initialize(p)
+        s.advance(p.copy(), 5)
+        before = s.state()
+        s.resize(128, 64)                          # same size: state kept
+        s.resize(0, 50)                            # non-positive: ignored
+        assert np.array_equal(s.state(), before)
+        small = s.getSimulationMemoryUsage()
+        s.resize(200, 100)
+        assert (s.getNx(), s.getNy()) == (200, 100)
+        q = P.euler_bow_shock(nx=200, ny=100, radius=14.0, center=(60.0, 50.0))
+        q.dt = 0.1
+        s.reset(q)
+        assert s.getSimulationMemoryUsage() > small
+        o = O.Oracle(q, fp64=True)
+        for l1, li in rel_errors(s.state().astype(np.float64), o.state()):
+            assert l1 < 1e-6
+        assert np.array_equal(s.getCellTypes(), o.cell_types())
+        s.advance(q.copy(), 20)
+        o.step(20, q.copy())
+        for l1, li in rel_errors(s.state64(), o.state()):
+            assert l1 < 1e-5
+
+
 @pytest.mark.parametrize("visc", [0, 1])
 def test_obstacle_update_mid_run(solver, visc):
     """updateObstacleGeometry (solver.cu:3171-3196) after 100 steps: the

@@ -6,6 +6,7 @@ import pytest
 
 import closed_forms as C
 import oracle as O
+from fvm_amd import params as P
 
 
 @pytest.mark.parametrize("viscous", [False, True])
@@ -99,4 +100,24 @@ def test_nonfinite_cell_stays_confined():
     assert spread == 0
     assert dev < 1e-13, dev
     assert np.abs(s1[0] - s0[0]).max() > 1e-2      # the neighbours did see it
+    o.close()
+
+
+def test_dual_energy_switch_stays_off():
+    """primitives (solver.cu:862-894) take e from rho_e only where
+    eta = rho_e / E < 1e-3.  For a uniform flow eta = 1 / (1 + gamma (gamma - 1)
+    M^2 / 2), 0.18 at Mach 4, so the branch is dead on these workloads; after
+    200 steps of the Mach-3 bow shock the lowest eta, in the expansion behind
+    the cylinder, is about 0.02, still 20 times above the switch."""
+    gamma = 1.4
+    assert 1.0 / (1.0 + gamma * (gamma - 1.0) * 4.0 ** 2 / 2.0) > 0.15
+    p = P.euler_bow_shock()
+    p.dt = 0.1
+    o = O.Oracle(p, fp64=False)
+    o.step(200, p.copy())
+    s = o.state()
+    fluid = o.cell_types() != P.CELL_SOLID
+    eta = s[4][fluid] / s[3][fluid]
+    assert int((eta < 1e-3).sum()) == 0
+    assert eta.min() > 1e-2
     o.close()
