@@ -217,6 +217,17 @@ size_t Engine::bytes_in_use() const {
     return b;
 }
 
+// Planes of the first `nsets` buffer sets: five each, except that set D (the
+// fourth) has no density plane of its own, since the first viscous half step
+// leaves rho unchanged and D reads A's (upload_desc).
+static size_t set_planes(int nsets) { return (size_t)5 * nsets - (nsets > 3 ? 1 : 0); }
+
+// Offset, in planes, of field f of buffer set s (set D's density is A's).
+static size_t plane_index(int s, int f) {
+    if (s < 3) return (size_t)s * 5 + f;
+    return f == 0 ? 0 : (size_t)15 + f - 1;
+}
+
 // Plane layout of one case inside its allocation.
 struct CaseLayout {
     size_t n, plane, sets_off, k_off, ct_off, sdf_off, slots_off, t_off, steps_off, tmax_off, total;
@@ -228,7 +239,7 @@ static CaseLayout layout_for(const fvm_params& p, bool fp64, bool kahan, int nse
     L.plane = round256(L.n * (fp64 ? 8 : 4));
     size_t off = 0;
     L.sets_off = off;
-    off += L.plane * 5 * nsets;
+    off += L.plane * set_planes(nsets);
     L.k_off = off;
     if (kahan) off += round256(L.n * 4) * 5;
     L.ct_off = off;
@@ -356,14 +367,13 @@ int Engine::upload_desc(HostCase& hc) {
     CaseLayout L = layout_for(hc.p, fp64_, kahan_, hc.nsets);
     CaseBufs b;
     std::memset(&b, 0, sizeof(b));
+    // The first viscous half step (A -> D) does not change rho, so D's density
+    // plane is A's (plane_index): that half step writes four fields, and the
+    // stage kernels and the blend read U^n's density in place.  A's density is
+    // next written by the final half step, after the blend, the last reader of D.
     for (int s = 0; s < 4; s++)
         for (int f = 0; f < 5; f++)
-            b.U[s][f] = (s < hc.nsets) ? (void*)(base + L.sets_off + L.plane * (s * 5 + f)) : nullptr;
-    // The first viscous half step (A -> D) does not change rho, so D's density
-    // plane is A's: that half step writes four fields, and the stage kernels
-    // and the blend read U^n's density in place.  A's density is next written
-    // by the final half step, after the blend, the last reader of D.
-    if (hc.nsets > 3) b.U[3][0] = b.U[0][0];
+            b.U[s][f] = (s < hc.nsets) ? (void*)(base + L.sets_off + L.plane * plane_index(s, f)) : nullptr;
     if (kahan_)
         for (int f = 0; f < 5; f++) b.K[f] = (float*)(base + L.k_off + round256(L.n * 4) * f);
     b.ct = (uint8_t*)(base + L.ct_off);
@@ -1045,7 +1055,7 @@ int Engine::read_state(int id, void* host, bool as_f64) {
     size_t es = fp64_ ? 8 : 4;
     std::vector<char> tmp;
     for (int f = 0; f < 5; f++) {
-        const char* src = (const char*)hc->mem + L.sets_off + L.plane * (cur * 5 + f);
+        const char* src = (const char*)hc->mem + L.sets_off + L.plane * plane_index(cur, f);
         if ((as_f64 && fp64_) || (!as_f64 && !fp64_)) {
             FVM_HIP(hipMemcpy((char*)host + f * L.n * es, src, L.n * es, hipMemcpyDeviceToHost));
         } else {
@@ -1073,7 +1083,7 @@ int Engine::write_state(int id, const void* host) {
     int cur = current_set(*hc);
     std::vector<double> tmp;
     for (int f = 0; f < 5; f++) {
-        char* dst = (char*)hc->mem + L.sets_off + L.plane * (cur * 5 + f);
+        char* dst = (char*)hc->mem + L.sets_off + L.plane * plane_index(cur, f);
         const float* s = (const float*)host + f * L.n;
         if (fp64_) {
             tmp.resize(L.n);
@@ -1272,7 +1282,7 @@ int Engine::save_case(int id, const char* path) {
     const int cur = current_set(*hc);
     std::vector<char> buf(L.n * h.elem_bytes);
     for (int f = 0; ok && f < 5; f++) {
-        FVM_HIP(hipMemcpy(buf.data(), base + L.sets_off + L.plane * (cur * 5 + f), buf.size(), hipMemcpyDeviceToHost));
+        FVM_HIP(hipMemcpy(buf.data(), base + L.sets_off + L.plane * plane_index(cur, f), buf.size(), hipMemcpyDeviceToHost));
         ok = std::fwrite(buf.data(), 1, buf.size(), out.f) == buf.size();
     }
     for (int f = 0; ok && kahan_ && f < 5; f++) {
@@ -1303,7 +1313,7 @@ int Engine::load_case(int id, const char* path) {
     for (int f = 0; f < 5; f++) {
         if (std::fread(buf.data(), 1, buf.size(), in.f) != buf.size())
             return set_error(FVM_ERR_INVALID_ARGUMENT, "snapshot state truncated");
-        FVM_HIP(hipMemcpy(base + L.sets_off + L.plane * (cur * 5 + f), buf.data(), buf.size(), hipMemcpyHostToDevice));
+        FVM_HIP(hipMemcpy(base + L.sets_off + L.plane * plane_index(cur, f), buf.data(), buf.size(), hipMemcpyHostToDevice));
     }
     for (int f = 0; kahan_ && f < 5; f++) {
         if (std::fread(buf.data(), 1, L.n * 4, in.f) != L.n * 4)
