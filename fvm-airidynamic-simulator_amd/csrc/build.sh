@@ -23,3 +23,8 @@ echo "built $OUT/libfvm_amd.so"
 g++ -std=c++17 -O2 -I "$HERE/../../include" "$HERE/../apps/kernel_benchmark.cpp" -o "$OUT/kernel_benchmark" \
     -L "$OUT" -lfvm_amd -Wl,-rpath,'$ORIGIN' -Wl,-rpath,/opt/rocm/lib
 echo "built $OUT/kernel_benchmark"
+# the reference front end's call order through the C++ drop-in (drop-in GPU test)
+g++ -std=c++17 -O2 -D__HIP_PLATFORM_AMD__ -I "$HERE/../../include" -I /opt/rocm/include \
+    "$HERE/../apps/front_end_sequence.cpp" -o "$OUT/front_end_sequence" \
+    -L "$OUT" -lfvm_amd -L /opt/rocm/lib -lamdhip64 -Wl,-rpath,'$ORIGIN' -Wl,-rpath,/opt/rocm/lib
+echo "built $OUT/front_end_sequence"

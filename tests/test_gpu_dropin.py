@@ -1,12 +1,13 @@
 """The C++ drop-in (include/fvm/solver.hpp) and the KernelBenchmark counterpart
 on the GPU.
 
-A C++ program compiled here calls CFDSolver in the order of the reference
-front end (main.cpp:203-368): initialize, getCellTypes, reset,
-convertCellTypesToDevice, updateObstacleGeometry, computeStableTimeStep, step,
-the *ToDevice readbacks into caller-owned device memory, queue/read dt, the
-maxima and the memory figure.  The same sequence through the Python binding
-must give the same numbers, and the readbacks must agree with each other.
+apps/front_end_sequence.cpp (built beside the library) calls CFDSolver in the
+order of the reference front end (main.cpp:203-368): initialize, getCellTypes,
+reset, convertCellTypesToDevice, updateObstacleGeometry, computeStableTimeStep,
+step, the *ToDevice readbacks into caller-owned device memory, queue/read dt,
+the maxima and the memory figure.  The same sequence through the Python
+binding must give the same numbers, and the readbacks must agree with each
+other.
 """
 import os
 import subprocess
@@ -20,79 +21,9 @@ from fvm_amd.lib import LIB_PATH
 
 pytestmark = pytest.mark.gpu
 
-REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBDIR = os.path.dirname(LIB_PATH)
 NSTEPS = 20
 
-FRONT_END = r'''
-#include <hip/hip_runtime_api.h>
-#include <cstdio>
-#include <vector>
-#include "fvm/solver.hpp"
-
-static void host_copy(std::vector<float>& h, const float* d) {
-    if (hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) std::exit(4);
-}
-
-int main() {
-    SimParams params;
-    params.nx = 256;
-    params.ny = 128;
-    params.enable_viscosity = true;
-    params.computeDerived();
-    const size_t n = (size_t)params.nx * params.ny;
-
-    CFDSolver solver;
-    solver.initialize(params);
-    std::vector<uint8_t> ct(n);
-    solver.getCellTypes(ct.data());
-    solver.reset(params);
-
-    float* dev = nullptr;
-    if (hipMalloc(&dev, n * sizeof(float)) != hipSuccess) return 4;
-    std::vector<float> h(n);
-    solver.convertCellTypesToDevice(dev);
-    host_copy(h, dev);
-    long mismatch = 0, solid = 0;
-    for (size_t i = 0; i < n; i++) { mismatch += h[i] != (float)ct[i]; solid += ct[i] == 1; }
-    std::printf("ct_mismatch %ld\nsolid %ld\n", mismatch, solid);
-
-    params.obstacle_x += 0.3f;
-    solver.updateObstacleGeometry(params);
-    std::vector<uint8_t> ct2(n);
-    solver.getCellTypes(ct2.data());
-    long moved = 0;
-    for (size_t i = 0; i < n; i++) moved += ct2[i] != ct[i];
-    std::printf("moved %ld\n", moved);
-
-    params.dt = solver.computeStableTimeStep(params);
-    std::printf("dt %.9e\n", params.dt);
-    for (int s = 0; s < %NSTEPS%; s++) solver.step(params);
-    solver.synchronize();
-    std::printf("step %d\nt %.9e\n", params.step, params.t_current);
-    solver.queueTimeStepComputation(params);
-    std::printf("dt_next %.9e\n", solver.readTimeStepResult(params));
-
-    const char* names[5] = {"T", "p", "rho", "vmag", "mach"};
-    for (int f = 0; f < 5; f++) {
-        switch (f) {
-            case 0: solver.computeTemperatureToDevice(dev); break;
-            case 1: solver.computePressureToDevice(dev); break;
-            case 2: solver.computeDensityToDevice(dev); break;
-            case 3: solver.computeVelocityMagToDevice(dev); break;
-            default: solver.computeMachToDevice(dev); break;
-        }
-        host_copy(h, dev);
-        double sum = 0; float mx = h[0];
-        for (size_t i = 0; i < n; i++) { sum += h[i]; mx = h[i] > mx ? h[i] : mx; }
-        std::printf("sum_%s %.12e\nmax_%s %.9e\n", names[f], sum, names[f], mx);
-    }
-    std::printf("maxT %.9e\nmaxMach %.9e\n", solver.getMaxTemperature(), solver.getMaxMach());
-    std::printf("memory %zu\n", solver.getSimulationMemoryUsage());
-    (void)hipFree(dev);
-    return 0;
-}
-'''.replace("%NSTEPS%", str(NSTEPS))
 
 
 def run(cmd, **kw):
@@ -130,14 +61,10 @@ def python_sequence():
         return dt, {f: s.field(f).astype(np.float64) for f in range(5)}, s.state(), stats
 
 
-def test_cpp_front_end_sequence(tmp_path):
-    src = tmp_path / "front_end.cpp"
-    exe = tmp_path / "front_end"
-    src.write_text(FRONT_END)
-    subprocess.check_call(["g++", "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(REPO, "include"),
-                           "-I", "/opt/rocm/include", str(src), "-o", str(exe), "-L", LIBDIR, "-lfvm_amd",
-                           "-L", "/opt/rocm/lib", "-lamdhip64", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,/opt/rocm/lib"])
-    r = run([str(exe)])
+def test_cpp_front_end_sequence():
+    exe = os.path.join(LIBDIR, "front_end_sequence")
+    assert os.path.exists(exe), "build the library first (it builds front_end_sequence beside it)"
+    r = run([exe, str(NSTEPS)])
     assert r.returncode == 0, r.stdout + r.stderr
     out = {k: float(v) for k, v in (l.split() for l in r.stdout.splitlines())}
     print(r.stdout, flush=True)
