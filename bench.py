@@ -61,11 +61,12 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(nx, ny):
     """HBM bytes per launch of the dominant kernel from the committed rocprof
-    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor).
+    The summary covers the 2048x1024 workload only; other grids get None."""
     path = os.path.join(REPO, "profiles", "stage_kernels_2048x1024_pmc.json")
-    if not os.path.exists(path):
+    if (nx, ny) != (2048, 1024) or not os.path.exists(path):
         return None
     try:
         with open(path) as f:
@@ -262,7 +263,7 @@ def main():
     avg_ms = kt[dominant][0]
     alg_bytes = KERNEL_BYTES_PER_CELL[dominant] * cells
     achieved = alg_bytes / (avg_ms * 1e-3)
-    tr = traffic_from_profiles()
+    tr = traffic_from_profiles(a.nx, a.ny)
     traffic = tr.get(dominant) if isinstance(tr, dict) else None
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dominant,
