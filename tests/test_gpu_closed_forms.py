@@ -117,3 +117,38 @@ def test_nonfinite_cell_stays_confined(solver, precision):
     assert spread == 0
     assert dev < (1e-7 if precision == "fp64" else 1e-6), dev
     assert np.abs(s1[0] - s0[0]).max() > 1e-2
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("viscous", [False, True])
+def test_freestream_preserved(solver, viscous, precision):
+    """Uniform flow with no obstacle stays uniform (computeFluxesKernel and
+    updateKernel, solver.cu:1450-1876; the viscous terms vanish).  Away from
+    the edges every face carries the same flux, so rho, rho_u, rho_v do not
+    change by a single bit; E and rho_e are re-derived from each other every
+    stage (dual-energy sync) and may move by an ulp.  The edges take the
+    characteristic boundary state through p / rho^gamma, which FP32 rounds
+    (exp and log) to ~1e-6."""
+    if viscous:
+        p = P.viscous_cylinder(nx=96, ny=48, diameter=8.0, reynolds=50.0, mach=0.5)
+    else:
+        p = P.euler_bow_shock(nx=96, ny=48)
+    C.no_obstacle(p)
+    p.dt = 0.1
+    p.precision = PRECISIONS[precision]
+    announce(f"freestream viscous={viscous} {precision} 96x48 50 steps")
+    solver.initialize(p)
+    s0 = solver.state64()
+    solver.advance(p.copy(), 50)
+    s1 = solver.state64()
+    dev = [float(np.abs(s1[c] - s0[c]).max() / max(np.abs(s0[c]).max(), 1.0)) for c in range(5)]
+    inner = (slice(12, -12), slice(12, -12))
+    dev_in = [float(np.abs(s1[c][inner] - s0[c][inner]).max() / max(np.abs(s0[c]).max(), 1.0)) for c in range(5)]
+    print(f"  largest change {max(dev):.1e}, interior {max(dev_in):.1e}", flush=True)
+    if precision == "fp64":
+        assert max(dev) < 1e-13, dev
+    else:
+        assert max(dev) < 2e-5, dev
+        for c in range(3):
+            assert np.array_equal(s1[c][inner], s0[c][inner]), c
+        assert max(dev_in[3:]) < 3e-7, dev_in
