@@ -53,7 +53,9 @@ enum {
     FVM_SHAPE_STARSHIP = 5
 };
 
-/* Riemann flux (addition: the reference always uses HLLC, solver.cu:1571). */
+/* Riemann flux (addition: the reference always uses HLLC, solver.cu:1571).
+ * AUSM+ (Liou 1996) has no low-Mach pressure diffusion: use it for supersonic
+ * flow; past walls at low Mach (e.g. Mach 0.3) it diverges with this scheme. */
 enum { FVM_FLUX_HLLC = 0, FVM_FLUX_AUSM_PLUS = 1 };
 /* MUSCL limiter (addition: the reference has minmod only, solver.cu:110). */
 enum { FVM_LIMITER_MINMOD = 0, FVM_LIMITER_VAN_LEER = 1 };
