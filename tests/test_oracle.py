@@ -36,11 +36,12 @@ def sod_params(nx, ny, length, along_y=False):
     return p
 
 
-def run_sod(along_y):
+def run_sod(along_y, flux=P.FLUX_HLLC):
     # domain [0, 2] along the tube, discontinuity at 1, t = 0.2 (Toro test 1)
     n_long, n_cross = 400, 160
     nx, ny = (n_cross, n_long) if along_y else (n_long, n_cross)
     p = sod_params(nx, ny, 2.0, along_y)
+    p.flux_scheme = flux
     h = p.dx
     o = O.Oracle(p, fp64=True)
     s = o.state()
@@ -67,15 +68,16 @@ def run_sod(along_y):
     return xs, r, un, ut, line
 
 
+@pytest.mark.parametrize("flux", [P.FLUX_HLLC, P.FLUX_AUSM_PLUS])
 @pytest.mark.parametrize("along_y", [False, True])
-def test_sod_exact_solution(along_y):
-    xs, r, un, ut, line = run_sod(along_y)
+def test_sod_exact_solution(along_y, flux):
+    xs, r, un, ut, line = run_sod(along_y, flux)
     ex = sample(xs, 0.2, 1.0, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1))
     win = (xs > 0.6) & (xs < 1.4)
     h = xs[1] - xs[0]
     l1_rho = np.sum(np.abs(r[win] - ex[0][win])) * h
     l1_u = np.sum(np.abs(un[win] - ex[1][win])) * h
-    # second-order MUSCL + HLLC on 400 cells: L1 errors of a few 1e-3 (shock and contact smeared)
+    # second-order MUSCL + HLLC or AUSM+ on 400 cells: L1 errors of a few 1e-3 (shock and contact smeared)
     assert l1_rho < 6e-3, l1_rho
     assert l1_u < 8e-3, l1_u
     # u = u* between the rarefaction tail (x~0.99) and the shock (x~1.35)
