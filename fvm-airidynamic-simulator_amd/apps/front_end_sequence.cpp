@@ -63,6 +63,9 @@ int main(int argc, char* argv[]) {
 
     const char* names[5] = {"T", "p", "rho", "vmag", "mach"};
     for (int f = 0; f < 5; f++) {
+        // a clear of the buffer still queued on the null stream must land
+        // before the readback, not after it
+        if (hipMemsetAsync(dev, 0, n * sizeof(float), nullptr) != hipSuccess) return 4;
         switch (f) {
             case 0: solver.computeTemperatureToDevice(dev); break;
             case 1: solver.computePressureToDevice(dev); break;
