@@ -26,6 +26,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "fvm-airidynamic-simulator_amd"))
 
 HBM_PEAK = 8.0e12  # bytes/s, MI355X
+# Vector-ALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, one FP32 lane
+# operation per lane per cycle (no packed math), i.e. a wave64 instruction
+# every 4 cycles per SIMD.
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
 # Algorithmic HBM bytes per cell for each kernel of one step (FP32 state of
 # 5 fields = 20 B, cell tag 1 B): what the kernel must read and write once.
 KERNEL_BYTES_PER_CELL = {
@@ -48,7 +52,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--batch-cases", type=int, default=512)
-    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-steps", type=int, default=2000, help="steps of every case (the mix specifies 2000)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=8.0,
                     help="requests/s of the served stream (8/s offers ~8.3e9 cell-updates/s, about 75%% of capacity)")
@@ -61,19 +65,19 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(nx, ny):
-    """HBM bytes per launch of the dominant kernel from the committed rocprof
-    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor).
-    The summary covers the 2048x1024 workload only; other grids get None."""
+def counters_from_profiles(nx, ny):
+    """Per-launch counters of the step kernels from the committed rocprof PMC
+    summary: HBM bytes (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
+    factor) and VALU instructions (SQ_INSTS_VALU).  The summary covers the
+    2048x1024 workload only; other grids get {}."""
     path = os.path.join(REPO, "profiles", "stage_kernels_2048x1024_pmc.json")
     if (nx, ny) != (2048, 1024) or not os.path.exists(path):
-        return None
+        return {}
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d.get("hbm_bytes_per_launch", {})
+            return json.load(f)
     except Exception:
-        return None
+        return {}
 
 
 def cpu_baseline(p, nsteps):
@@ -185,10 +189,13 @@ def batched_vs_plain(ncases, nsteps):
     from fvm_amd.batch import BatchEngine
     cases = P.mixed_batch(ncases)
     cells = sum(p.nx * p.ny for p in cases)
+    # both sides capture their step graphs and take 2 warm-up steps before
+    # their timed windows, so no capture is timed on either side
     b = BatchEngine(0, mem_budget_bytes=16 << 30, max_cases=ncases)
     for p in cases:
         b.add(p)
-    b.advance(2)  # capture the step graph
+    b.prepare_graphs()
+    b.advance(2)
     b.synchronize()
     t0 = time.perf_counter()
     b.advance(nsteps)
@@ -200,6 +207,7 @@ def batched_vs_plain(ncases, nsteps):
     for p in cases:
         s.initialize(p)
         q = p.copy()
+        s.prepare_graphs(q)
         s.advance(q, 2)
         s.synchronize()
         t0 = time.perf_counter()
@@ -243,6 +251,9 @@ def main():
     s = fvm_amd.CFDSolver(0)
     s.initialize(p)
     q = p.copy()
+    # the 2- and 16-step graphs the timed steps replay are captured here, so
+    # whatever --warmup is, the timed window holds no graph capture
+    s.prepare_graphs(q)
     s.advance(q, a.warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -263,14 +274,23 @@ def main():
     avg_ms = kt[dominant][0]
     alg_bytes = KERNEL_BYTES_PER_CELL[dominant] * cells
     achieved = alg_bytes / (avg_ms * 1e-3)
-    tr = traffic_from_profiles(a.nx, a.ny)
-    traffic = tr.get(dominant) if isinstance(tr, dict) else None
+    prof = counters_from_profiles(a.nx, a.ny)
+    traffic = prof.get("hbm_bytes_per_launch", {}).get(dominant)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dominant,
-                "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": alg_bytes,
-                "note": "the stage kernels issue ~800 VALU instructions per cell and run at ~80% VALU busy "
-                        "(rocprof VALUBusy): vector-ALU bound; HBM traffic is 4% (stage 1) to 10% (stage 2) "
-                        "above the algorithmic bytes"}
+                "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": alg_bytes}
+    # The roof that binds: vector-ALU issue.  SQ_INSTS_VALU (wave instructions
+    # per launch, from the committed PMC summary of this build) x 64 lanes over
+    # the same live launch duration, against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz.
+    valu = prof.get("valu_insts_per_dispatch", {}).get(dominant)
+    if valu:
+        lane_ops = valu * 64 / (avg_ms * 1e-3)
+        roofline["binding"] = "valu"
+        roofline["valu"] = {"achieved": lane_ops / 1e12, "peak": VALU_PEAK / 1e12, "unit": "lane-Tops/s",
+                            "frac": lane_ops / VALU_PEAK, "insts_per_launch": valu}
+        roofline["note"] = ("the stage kernels are vector-ALU issue bound: valu.frac is their VALU issue "
+                            "fraction at the measured duration; HBM traffic is within ~10% of the "
+                            "algorithmic bytes")
 
     t_sim, nsteps_dev = s.time()
     out = {

@@ -277,6 +277,15 @@ int fvm_advance(fvm_solver* s, fvm_params* p, int nsteps) {
     return FVM_OK;
 }
 
+int fvm_prepare_graphs(fvm_solver* s, const fvm_params* p) {
+    CHECK_SOLVER(s);
+    if (!p) return set_error(FVM_ERR_INVALID_ARGUMENT, "null params");
+    int rc = apply_params(s, *p);
+    if (rc) return rc;
+    if (p->dt_mode != FVM_DT_DEVICE_CFL) return FVM_OK;   // host dt: fvm_advance launches directly
+    return s->eng->prepare_graphs();
+}
+
 int fvm_compute_stable_dt(fvm_solver* s, const fvm_params* p, float* dt) {
     CHECK_SOLVER(s);
     if (!p || !dt) return set_error(FVM_ERR_INVALID_ARGUMENT, "null argument");
@@ -545,6 +554,12 @@ int fvm_batch_advance(fvm_batch* b, int32_t nsteps) {
         e.retire_finished();
     }
     return FVM_OK;
+}
+
+int fvm_batch_prepare_graphs(fvm_batch* b) {
+    if (!b) return set_error(FVM_ERR_INVALID_ARGUMENT, "null batch");
+    if (!b->use_graph || b->eng->active_count() == 0) return FVM_OK;
+    return b->eng->prepare_graphs();
 }
 
 int fvm_batch_case_status(fvm_batch* b, int32_t case_id, int32_t* steps_done, double* t, int32_t* finished) {

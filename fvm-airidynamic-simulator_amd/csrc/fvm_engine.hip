@@ -1036,6 +1036,23 @@ int Engine::step_graph(int nsteps) {
     return FVM_OK;
 }
 
+int Engine::prepare_graphs() {
+    if (lists_dirty_) {
+        int rc = rebuild_lists();
+        if (rc) return rc;
+    }
+    if (timing_) return FVM_OK;   // timed runs launch directly
+    if (!graph_valid_) {
+        int rc = capture_graphs();
+        if (rc) return rc;
+    }
+    if (!long_valid_) {
+        int rc = capture_long_graphs();
+        if (rc) return rc;
+    }
+    return FVM_OK;
+}
+
 int Engine::advance(int nsteps, bool use_graph) {
     if (nsteps <= 0) return FVM_OK;
     return use_graph ? step_graph(nsteps) : step_direct(nsteps);

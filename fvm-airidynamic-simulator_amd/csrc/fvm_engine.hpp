@@ -94,6 +94,10 @@ public:
     int step_direct(int nsteps);        // launches without a graph
     int step_graph(int nsteps);         // replays captured 16- and 2-step graphs
     int advance(int nsteps, bool use_graph);
+    // Builds the work lists and captures the step graphs that step_graph
+    // replays (both parities, 2 and LONG_STEPS steps) without stepping, so a
+    // later advance only launches.
+    int prepare_graphs();
     int sync();
 
     // per-case access

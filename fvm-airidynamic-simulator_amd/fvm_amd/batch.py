@@ -53,6 +53,10 @@ class BatchEngine:
     def advance(self, nsteps: int):
         check(lib().fvm_batch_advance(self._h, nsteps), "fvm_batch_advance")
 
+    def prepare_graphs(self):
+        """Capture the step graphs of the current cases without stepping."""
+        check(lib().fvm_batch_prepare_graphs(self._h), "fvm_batch_prepare_graphs")
+
     def status(self, cid: int):
         n = ctypes.c_int32()
         t = ctypes.c_double()

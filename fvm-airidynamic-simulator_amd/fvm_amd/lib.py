@@ -48,6 +48,7 @@ SIGNATURES = [
     ("fvm_update_obstacle", _i, [_vp, _P]),
     ("fvm_step", _i, [_vp, _P]),
     ("fvm_advance", _i, [_vp, _P, _i]),
+    ("fvm_prepare_graphs", _i, [_vp, _P]),
     ("fvm_compute_stable_dt", _i, [_vp, _P, _fp]),
     ("fvm_queue_dt", _i, [_vp, _P]),
     ("fvm_read_dt", _i, [_vp, _P, _fp]),
@@ -78,6 +79,7 @@ SIGNATURES = [
     ("fvm_batch_case_bytes", _i, [_P, _szp]),
     ("fvm_batch_remove_case", _i, [_vp, ctypes.c_int32]),
     ("fvm_batch_advance", _i, [_vp, ctypes.c_int32]),
+    ("fvm_batch_prepare_graphs", _i, [_vp]),
     ("fvm_batch_case_status", _i, [_vp, ctypes.c_int32, _i32p, _dp, _i32p]),
     ("fvm_batch_get_state", _i, [_vp, ctypes.c_int32, _fp]),
     ("fvm_batch_get_cell_types", _i, [_vp, ctypes.c_int32, _u8p]),

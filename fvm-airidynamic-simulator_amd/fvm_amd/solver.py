@@ -65,6 +65,10 @@ class CFDSolver:
     def advance(self, params: P.FvmParams, nsteps: int):
         check(lib().fvm_advance(self._h, ctypes.byref(params), int(nsteps)), "fvm_advance")
 
+    def prepare_graphs(self, params: P.FvmParams):
+        """Capture the step graphs advance() replays, without stepping."""
+        check(lib().fvm_prepare_graphs(self._h, ctypes.byref(params)), "fvm_prepare_graphs")
+
     # -- CFL dt (solver.cu:3206-3278)
     def computeStableTimeStep(self, params: P.FvmParams) -> float:
         dt = ctypes.c_float()

@@ -165,6 +165,11 @@ int fvm_step(fvm_solver* s, fvm_params* p);
  * replayed from a captured HIP graph and dt never leaves the device.
  * params->t_current / step are updated at the end. */
 int fvm_advance(fvm_solver* s, fvm_params* p, int nsteps);
+/* Captures the HIP graphs fvm_advance replays with FVM_DT_DEVICE_CFL (2 and
+ * 16 steps, both step parities) for params *p without stepping, so the next
+ * fvm_advance only launches.  A no-op with FVM_DT_HOST.  No reference
+ * counterpart (the reference launches every kernel of every step). */
+int fvm_prepare_graphs(fvm_solver* s, const fvm_params* p);
 /* computeStableTimeStep (solver.cu:3273-3278). */
 int fvm_compute_stable_dt(fvm_solver* s, const fvm_params* p, float* dt);
 /* queueTimeStepComputation / readTimeStepResult (solver.cu:3206-3265). */
@@ -281,6 +286,9 @@ int fvm_batch_remove_case(fvm_batch* b, int32_t case_id);
 /* Advance every active, unfinished case by up to nsteps steps; each case uses
  * its own CFL dt (FVM_DT_DEVICE_CFL) or its params.dt (FVM_DT_HOST). */
 int fvm_batch_advance(fvm_batch* b, int32_t nsteps);
+/* Captures the step graphs of the current case set without stepping (see
+ * fvm_prepare_graphs); admitting or retiring cases keeps them valid. */
+int fvm_batch_prepare_graphs(fvm_batch* b);
 /* Case progress: steps done, simulated time, finished flag. */
 int fvm_batch_case_status(fvm_batch* b, int32_t case_id, int32_t* steps_done, double* t, int32_t* finished);
 int fvm_batch_get_state(fvm_batch* b, int32_t case_id, float* host5);

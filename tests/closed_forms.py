@@ -27,6 +27,15 @@ relative to the field's scale.
 - Sutherland's law (2317-2319) through the CFL dt (3206-3278).
 - A non-finite cell (updateKernel's fallbacks, 1812-1821): NaN reaches no
   other cell, and nothing outside its domain of dependence changes.
+- Heat conduction (2352, 2414-2415): fluid at rest at uniform pressure with a
+  quadratic temperature profile T = T0 (1 + b (y - y0)^2).  The convective
+  step leaves such a state alone, so one viscous step changes E and rho_e by
+  dt d/dy(k dT/dy), k = mu(T) cp / Pr, and nothing else.
+- Dilatational stress (2395-2396): the isotropic expansion u = a (x - x0),
+  v = a (y - y0) at uniform T has tau_xx = tau_yy = mu (2a - (2/3) 2a) and
+  tau_xy = 0, so the viscous terms add dt (4/3) mu a^2 to E (work of the
+  stress) and to rho_e (dissipation) and leave the momentum alone.  The
+  increment is the viscous run minus the inviscid run from the same state.
 """
 import math
 
@@ -335,3 +344,115 @@ def nonfinite_spread(s0, s1, nsteps):
     ref = s0[:, far]
     scale = np.maximum(np.abs(ref).max(axis=1, keepdims=True), 1.0)
     return int(bad.sum()), float((np.abs(s1[:, far] - ref) / scale).max())
+
+
+# ---------------------------------------------------------------- viscous terms
+
+VISCOUS_STEP_DT = 0.05
+VISCOUS_MU_REF = 2.0
+VISCOUS_MARGIN = 10    # cells from the domain edges that the boundary pass reaches in one step
+
+
+def viscous_rest_params(nx, ny):
+    """Viscous parameters of a fluid at rest (Mach 0, no obstacle) with
+    mu_ref = 2, host dt 0.05 (cfl_visc dx^2 / nu is about 0.19 here)."""
+    p = P.viscous_cylinder(nx=nx, ny=ny, diameter=8.0, reynolds=20.0, mach=0.0)
+    no_obstacle(p)
+    p.mu_ref = VISCOUS_MU_REF
+    p.dt_mode = P.DT_HOST
+    p.dt = VISCOUS_STEP_DT
+    return p
+
+
+def sutherland_mu(p, T):
+    """mu = mu_ref (T/T_ref)^(3/2) (T_ref + S) / (T + S) and its T derivative
+    mu (3/(2T) - 1/(T + S))."""
+    tr, s = p.t_ref, p.s_sutherland
+    mu = p.mu_ref * (T / tr) ** 1.5 * (tr + s) / (T + s)
+    return mu, mu * (1.5 / T - 1.0 / (T + s))
+
+
+HEAT_B = 2.2e-4     # T = T_inf (1 + b (y - y0)^2): 22% warmer at the top and bottom edges
+
+
+def heat_conduction_case(f32=True):
+    """48x64 fluid at rest at p_inf, T = T_inf (1 + b (y - 32)^2) (so rho =
+    p / (R T)); uniform in x."""
+    p = viscous_rest_params(48, 64)
+    X, Y = grid(p)
+    T = p.T_inf * (1.0 + HEAT_B * (Y - 32.0) ** 2)
+    pr = np.full_like(T, p.p_inf)
+    rho = pr / (p.r_gas * T)
+    zero = np.zeros_like(T)
+    return p, conservative(p, rho, zero, zero, pr, f32)
+
+
+def heat_conduction_expected(p):
+    """dt d/dy(k dT/dy) at the cell centres, k = mu(T) cp / Pr:
+    dt (k'(T) (dT/dy)^2 + k(T) d2T/dy2) with dT/dy = 2 b T_inf (y - y0),
+    d2T/dy2 = 2 b T_inf; returns (ny, nx)."""
+    X, Y = grid(p)
+    T = p.T_inf * (1.0 + HEAT_B * (Y - 32.0) ** 2)
+    cp = p.gamma * p.r_gas / (p.gamma - 1.0)
+    mu, dmu = sutherland_mu(p, T)
+    k, dk = mu * cp / p.prandtl, dmu * cp / p.prandtl
+    dT = 2.0 * HEAT_B * p.T_inf * (Y - 32.0)
+    return p.dt * (dk * dT * dT + k * 2.0 * HEAT_B * p.T_inf)
+
+
+EXPANSION_A = 0.02
+
+
+def expansion_case(f32=True):
+    """64x64 isotropic expansion u = a (x - 32), v = a (y - 32) at uniform
+    rho_inf, p_inf (uniform T)."""
+    p = viscous_rest_params(64, 64)
+    X, Y = grid(p)
+    rho = np.full_like(X, p.rho_inf)
+    pr = np.full_like(X, p.p_inf)
+    return p, conservative(p, rho, EXPANSION_A * (X - 32.0), EXPANSION_A * (Y - 32.0), pr, f32)
+
+
+def expansion_expected(p):
+    """dt (4/3) mu(T_inf) a^2: the increment of E (div(tau . v)) and of rho_e
+    (the dissipation tau : grad v) of the isotropic expansion."""
+    mu, _ = sutherland_mu(p, p.T_inf)
+    return p.dt * (4.0 / 3.0) * mu * EXPANSION_A ** 2
+
+
+EXPANSION_HALF = 6   # the check region: cells within 6 of the centre in x and y
+
+
+def viscous_increment(p, s_visc, s_inv, which):
+    """Viscous increment of one step (viscous run minus inviscid run) on the
+    check region of a case: `which` = "heat" (the interior rows and columns)
+    or "expansion" (the centre block)."""
+    d = s_visc - s_inv
+    if which == "heat":
+        m = VISCOUS_MARGIN
+        return d[:, m:p.ny - m, m:p.nx - m]
+    c, h = 32, EXPANSION_HALF
+    return d[:, c - h:c + h, c - h:c + h]
+
+
+def heat_conduction_error(p, s_visc, s_inv):
+    """(worst |dE / expected - 1|, worst |d rho_e / expected - 1|, largest
+    change of rho and of the momentum relative to the E increment) on the
+    interior cells."""
+    d = viscous_increment(p, s_visc, s_inv, "heat")
+    m = VISCOUS_MARGIN
+    want = heat_conduction_expected(p)[m:p.ny - m, m:p.nx - m]
+    eE = float(np.abs(d[3] / want - 1.0).max())
+    ee = float(np.abs(d[4] / want - 1.0).max())
+    other = float(np.abs(d[:3]).max() / np.abs(want).max())
+    return eE, ee, other
+
+
+def expansion_error(p, s_visc, s_inv):
+    """(|mean dE / expected - 1|, |mean d rho_e / expected - 1|, worst cell
+    |dE / expected - 1|, largest momentum increment relative to the E
+    increment) over the centre block."""
+    d = viscous_increment(p, s_visc, s_inv, "expansion")
+    want = expansion_expected(p)
+    return (abs(float(d[3].mean()) / want - 1.0), abs(float(d[4].mean()) / want - 1.0),
+            float(np.abs(d[3] / want - 1.0).max()), float(np.abs(d[1:3]).max()) / want)

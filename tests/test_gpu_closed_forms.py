@@ -152,3 +152,49 @@ def test_freestream_preserved(solver, viscous, precision):
         for c in range(3):
             assert np.array_equal(s1[c][inner], s0[c][inner]), c
         assert max(dev_in[3:]) < 3e-7, dev_in
+
+
+def viscous_and_inviscid(solver, case, precision):
+    """One step of a viscous closed-form case from the same float32 state,
+    with and without viscosity, through the C ABI."""
+    p, s0 = case()
+    out = []
+    for visc in (1, 0):
+        q = p.copy()
+        q.enable_viscosity = visc
+        start(solver, q, s0, precision)
+        solver.advance(q.copy(), 1)
+        out.append(solver.state64())
+    return p, out[0], out[1]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_heat_conduction_closed_form(solver, precision):
+    """Heat conduction k = mu cp / Pr, q = -k grad T (solver.cu:2352,
+    2414-2415) in the viscous half steps: one step of the quadratic
+    temperature profile adds dt d/dy(k dT/dy) to E and rho_e.  FP32: the
+    increment is ~1e-4 of E, so the state's rounding bounds it to ~1e-3."""
+    announce(f"heat conduction {precision} 48x64 1 step")
+    p, sv, si = viscous_and_inviscid(solver, C.heat_conduction_case, precision)
+    eE, ee, other = C.heat_conduction_error(p, sv, si)
+    print(f"  dE {eE:.1e}, d rho_e {ee:.1e}, other fields {other:.1e}", flush=True)
+    tol = 2e-3 if precision == "fp64" else 5e-3
+    assert eE < tol, eE
+    assert ee < tol, ee
+    assert other < 1e-3, other
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_dilatational_stress_closed_form(solver, precision):
+    """tau_xx = mu (2 du/dx - (2/3) div v) (solver.cu:2395-2396): the viscous
+    increment of E and rho_e on the isotropic expansion is dt (4/3) mu a^2
+    (twice that with 1/3 for 2/3).  FP32: the increment is ~3e-5 of E, so
+    single cells carry ~2e-3 of rounding; the block mean is tighter."""
+    announce(f"isotropic expansion {precision} 64x64 1 step")
+    p, sv, si = viscous_and_inviscid(solver, C.expansion_case, precision)
+    mE, me, worst, mom = C.expansion_error(p, sv, si)
+    print(f"  mean dE {mE:.1e}, mean d rho_e {me:.1e}, worst cell {worst:.1e}, momentum {mom:.1e}", flush=True)
+    assert mE < 1e-2, mE
+    assert me < 1e-2, me
+    assert worst < (1e-2 if precision == "fp64" else 3e-2), worst
+    assert mom < (1e-3 if precision == "fp64" else 1e-2), mom

@@ -42,3 +42,47 @@ def test_admission_respects_budget():
     stats = srv.run(reqs)
     srv.close()
     assert stats["completed"] == 5
+
+
+@pytest.mark.timeout(900)
+def test_served_large_requests_vs_fp64_oracle():
+    """Requests at the served configuration's large end (1024x512 to
+    2048x1024, drawn as the served stream draws them) arrive while smaller
+    ones run; each large result is compared with the FP64 oracle per field,
+    with the FP32 oracle as the rounding floor, and each small one with its
+    solo run (bitwise)."""
+    import oracle as O
+    from parity import field_parity, oracle_state
+    rng = np.random.default_rng(31)
+    reqs = []
+    sizes = [(1024, 512), (1536, 768), (2048, 1024)]
+    for k, (nx, ny) in enumerate(sizes):
+        p = P.mixed_case(rng, n_lo=nx * ny, n_hi=nx * ny, nx_range=(nx, nx), ny_range=(ny, ny))
+        reqs.append(Request(k, arrival=0.002 * k, params=p, steps=int(rng.integers(60, 101))))
+    for k in range(3, 9):
+        p = P.mixed_case(rng, n_lo=128 * 64, n_hi=256 * 128, nx_range=(128, 256), ny_range=(64, 128))
+        reqs.append(Request(k, arrival=0.001 * k, params=p, steps=int(rng.integers(20, 120))))
+    print(f"\n[case] served: 3 large requests {sizes} among 6 small ones", flush=True)
+    srv = SolveServer(0, mem_budget_bytes=8 << 30, max_active=16, chunk_steps=10, keep_results=True)
+    stats = srv.run(reqs)
+    srv.close()
+    assert stats["completed"] == len(reqs)
+    done = {r.rid: r for r in stats["done"]}
+    # the large ones overlapped on the device with each other and with small ones
+    big = [done[k] for k in range(3)]
+    assert max(r.admitted for r in big) < min(r.completed for r in big)
+    s = fvm_amd.CFDSolver(0)
+    for r in stats["done"]:
+        p = r.params
+        print(f"  request {r.rid}: {p.nx}x{p.ny} shape {p.obstacle_shape} Mach {p.mach:.2f} "
+              f"viscous {p.enable_viscosity} {r.steps} steps", flush=True)
+        if r.rid < 3:
+            o = O.Oracle(p, fp64=True)
+            o.step(r.steps, p.copy())
+            field_parity(r.result.astype(np.float64), o.state(), oracle_state(p, r.steps), "    ")
+            o.close()
+        else:
+            s.initialize(p)
+            s.advance(p.copy(), r.steps)
+            assert np.array_equal(r.result, s.state()), f"request {r.rid}"
+    s.close()

@@ -121,3 +121,44 @@ def test_dual_energy_switch_stays_off():
     assert int((eta < 1e-3).sum()) == 0
     assert eta.min() > 1e-2
     o.close()
+
+
+def viscous_and_inviscid(case):
+    """One step of a viscous closed-form case from the same state, with and
+    without viscosity (FP64 oracle)."""
+    p, s0 = case(f32=False)
+    out = []
+    for visc in (1, 0):
+        q = p.copy()
+        q.enable_viscosity = visc
+        o = O.Oracle(q, fp64=True)
+        o.set_state(s0)
+        o.step(1, q.copy())
+        out.append(o.state())
+        o.close()
+    return p, out[0], out[1]
+
+
+def test_heat_conduction_closed_form():
+    """k = mu cp / Pr and q = -k grad T (solver.cu:2352, 2414-2415): one step of
+    the quadratic temperature profile adds dt d/dy(k dT/dy) to E and rho_e;
+    density and momentum do not change.  A doubled k, a dropped cp or Pr, or a
+    flipped sign of q is off by 50% or more."""
+    p, sv, si = viscous_and_inviscid(C.heat_conduction_case)
+    eE, ee, other = C.heat_conduction_error(p, sv, si)
+    assert eE < 2e-3, eE
+    assert ee < 2e-3, ee
+    assert other < 1e-3, other
+
+
+def test_dilatational_stress_closed_form():
+    """tau_xx = mu (2 du/dx - (2/3) div v) (solver.cu:2395-2396): on the
+    isotropic expansion the viscous increment of E and rho_e is
+    dt (4/3) mu a^2 (with 1/3 in place of 2/3 it would be twice that) and the
+    momentum gets none."""
+    p, sv, si = viscous_and_inviscid(C.expansion_case)
+    mE, me, worst, mom = C.expansion_error(p, sv, si)
+    assert mE < 1e-2, mE
+    assert me < 1e-2, me
+    assert worst < 1e-2, worst
+    assert mom < 1e-3, mom
