@@ -120,12 +120,20 @@ int apply_params(fvm_solver* s, const fvm_params& p) {
     return FVM_OK;
 }
 
-float dt_from_maxima(const fvm_params& p, float max_speed, float max_nu) {
-    // readTimeStepResult, src/solver.cu:3250-3265
+// readTimeStepResult, src/solver.cu:3250-3265: FP32 as in the reference, or
+// FP64 from the FP64 maxima (FP64 state and compensated variant), rounded to
+// the float the interface returns.
+float dt_from_maxima(const fvm_params& p, double max_speed, double max_nu, bool dt64) {
+    if (dt64) {
+        const double min_dx = std::fmin((double)p.dx, (double)p.dy);
+        double dt = (double)p.cfl * min_dx / (max_speed + (double)1e-10f);
+        if (p.enable_viscosity) dt = std::fmin(dt, (double)p.cfl_visc * min_dx * min_dx / (max_nu + (double)1e-10f));
+        return (float)dt;
+    }
     float min_dx = std::fmin(p.dx, p.dy);
-    float dt = p.cfl * min_dx / (max_speed + 1e-10f);
+    float dt = p.cfl * min_dx / ((float)max_speed + 1e-10f);
     if (p.enable_viscosity) {
-        float dtv = p.cfl_visc * min_dx * min_dx / (max_nu + 1e-10f);
+        float dtv = p.cfl_visc * min_dx * min_dx / ((float)max_nu + 1e-10f);
         dt = std::fmin(dt, dtv);
     }
     return dt;
@@ -289,10 +297,10 @@ int fvm_prepare_graphs(fvm_solver* s, const fvm_params* p) {
 int fvm_compute_stable_dt(fvm_solver* s, const fvm_params* p, float* dt) {
     CHECK_SOLVER(s);
     if (!p || !dt) return set_error(FVM_ERR_INVALID_ARGUMENT, "null argument");
-    float ms = 0, mn = 0;
+    double ms = 0, mn = 0;
     int rc = s->eng->read_maxima(s->case_id, &ms, &mn);
     if (rc) return rc;
-    *dt = dt_from_maxima(*p, ms, mn);
+    *dt = dt_from_maxima(*p, ms, mn, s->eng->dt64());
     return FVM_OK;
 }
 

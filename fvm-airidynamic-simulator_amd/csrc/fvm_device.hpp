@@ -32,7 +32,7 @@ struct CaseConsts {
     int noise_field;
     uint32_t noise_seed;
     float noise_amp;
-    float cfl, cfl_visc, min_dx, host_dt;   // dt arithmetic is FP32 as in the reference
+    float cfl, cfl_visc, min_dx, host_dt;
     T dx, dy, inv_dx, inv_dy, inv_2dx, inv_2dy;
     T gamma, gm1, inv_gm1, R, inv_R, cp, prandtl, two_over_gm1, gm1_over_4, inv_gamma, cs_fac;
     T min_rho, min_p, max_T, min_T, low_rho, vmax, vghost, slip_ref;
@@ -65,18 +65,25 @@ struct CaseBufs {
     void* ghosts;         // GhostInfo<T>[n_ghosts]
     int* gmap;            // ghost-table index of each cell of the ghost bounding box, -1 elsewhere
     int gbox[4];          // bounding box of the ghost cells: i0, j0, width, height
-    unsigned* slots;      // [2 slots][2]: bits of max wave speed, max mu/rho
+    unsigned* slots;      // [2 slots][2]: bits of max wave speed, max mu/rho (FP32 maxima)
     float* tile_max;      // [2][tiles]: per-tile max wave speed, max mu/rho of the step's final kernel
     double* t;            // simulated time
     long long* steps;     // steps taken
 };
 
+// The FP64 CFL maxima sit after the fields the stage kernels load, so that
+// those keep the offsets (and scalar-load grouping) they have without them.
 template <class T>
 struct CaseDesc {
     CaseConsts<T> c;
     CaseBufs b;
     int tiles_x, tiles_y;
-    int pad[2];
+    // 1: the CFL maxima and the dt formula in FP64 (FP64 state, and the
+    // compensated FP32 variant); 0: FP32 as in the reference
+    int dt64;
+    int pad;
+    double* slots64;      // CaseBufs::slots in FP64 (dt64 cases; atomics on the 64-bit patterns)
+    double* tile_max64;   // CaseBufs::tile_max in FP64 (dt64 cases)
 };
 
 // One work item: a tile of a case (stage / diffusion kernels) or a ghost cell.
@@ -441,18 +448,55 @@ __device__ __forceinline__ T wave_speed(const CaseConsts<T>& k, const T v[5], T*
     return Num<T>::sqrt(u * u + vv * vv) + c;
 }
 
-// dt from the maxima of the previous step (readTimeStepResult, solver.cu:3250-3265)
-template <class T>
-__device__ __forceinline__ float dt_from_slot(const CaseConsts<T>& k, const unsigned* slots, int slot) {
-    if (k.dt_mode == 0 || slots == nullptr) return k.host_dt;
-    float ms = __uint_as_float(slots[slot * 2 + 0]);
-    float mn = __uint_as_float(slots[slot * 2 + 1]);
-    float dt = k.cfl * k.min_dx / (ms + 1e-10f);
-    if (k.visc) {
-        float dtv = k.cfl_visc * k.min_dx * k.min_dx / (mn + 1e-10f);
-        dt = fminf(dt, dtv);
+// The same wave speed evaluated in arithmetic type A from a state of type T
+// (the FP64 CFL reduction of the compensated FP32 variant: the FP32 state is
+// widened and every operation is FP64; for T = A it is wave_speed).
+template <class A, class T>
+__device__ __forceinline__ A wave_speed_as(const CaseConsts<T>& k, const T v[5]) {
+    const A rho = (A)v[0], ru = (A)v[1], rv = (A)v[2], E = (A)v[3], re = (A)v[4];
+    const A r = tmax(rho, (A)k.min_rho);
+    const A ir = Num<A>::rcp(r);
+    const A u = ru * ir, vv = rv * ir;
+    const A ke = A(0.5) * r * (u * u + vv * vv);
+    const A e = (E > A(0) && re < A(0.001f) * E) ? re : E - ke;
+    const A p = tmax((A)k.gm1 * e, (A)k.min_p);
+    const A c = Num<A>::sqrt((A)k.gamma * p * Num<A>::rcp(rho + A(1e-10f)));
+    return Num<A>::sqrt(u * u + vv * vv) + c;
+}
+
+// dt from the maxima of the previous step (readTimeStepResult,
+// solver.cu:3250-3265): FP32 as in the reference, or (DT64: the FP64 state
+// and the compensated variant, whose engines set CaseDesc::dt64) FP64 from the FP64
+// maxima.  Returned in FP64; for FP32 it is the widened FP32 value, so the
+// FP32 arithmetic downstream is unchanged.
+template <bool DT64, class T>
+__device__ __forceinline__ double dt_from_slot(const CaseDesc<T>& cd, int slot) {
+    const CaseConsts<T>& k = cd.c;
+    const CaseBufs& b = cd.b;
+    if (k.dt_mode == 0 || b.slots == nullptr) return (double)k.host_dt;
+    if constexpr (DT64) {
+        const double ms = cd.slots64[slot * 2 + 0], mn = cd.slots64[slot * 2 + 1];
+        const double mdx = (double)k.min_dx;
+        double dt = (double)k.cfl * mdx / (ms + (double)1e-10f);
+        if (k.visc) dt = fmin(dt, (double)k.cfl_visc * mdx * mdx / (mn + (double)1e-10f));
+        return dt;
+    } else {
+        float ms = __uint_as_float(b.slots[slot * 2 + 0]);
+        float mn = __uint_as_float(b.slots[slot * 2 + 1]);
+        float dt = k.cfl * k.min_dx / (ms + 1e-10f);
+        if (k.visc) {
+            float dtv = k.cfl_visc * k.min_dx * k.min_dx / (mn + 1e-10f);
+            dt = fminf(dt, dtv);
+        }
+        return (double)dt;
     }
-    return dt;
+}
+
+// Atomic maximum of a non-negative value on its bit pattern (non-negative
+// IEEE values order like their bits, as unsigned integers of the same width).
+__device__ __forceinline__ void atomic_max_bits(unsigned* target, float v) { atomicMax(target, __float_as_uint(v)); }
+__device__ __forceinline__ void atomic_max_bits(double* target, double v) {
+    atomicMax((unsigned long long*)target, (unsigned long long)__double_as_longlong(v));
 }
 
 // Wave-wide max (64 lanes), then across the workgroup through LDS, then one
@@ -466,40 +510,46 @@ __device__ __forceinline__ void slot_max(unsigned* target, float m) {
     unsigned cur = __hip_atomic_load(target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (__float_as_uint(m) > cur) atomicMax(target, __float_as_uint(m));
 }
+__device__ __forceinline__ void slot_max(double* target, double m) {
+    if (!(m > 0.0)) return;
+    unsigned long long* t = (unsigned long long*)target;
+    unsigned long long cur = __hip_atomic_load(t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned long long)__double_as_longlong(m) > cur) atomic_max_bits(target, m);
+}
 
 // Workgroup maxima of NQ (<= 2) quantities stored for one tile (plain stores,
 // no atomics): tile_max[tile] and tile_max[ntiles + tile].  The step's last
 // kernel reduces a case's tiles into its CFL slot.  Called by the whole
 // workgroup; lds holds NQ x (NT / 64) floats.
-template <int NT, int NQ>
-__device__ __forceinline__ void block_max_store(const float (&v)[NQ], float (*lds)[NT / 64], float* tile_max,
+template <int NT, int NQ, class V>
+__device__ __forceinline__ void block_max_store(const V (&v)[NQ], V (*lds)[NT / 64], V* tile_max,
                                                 int ntiles, int tile) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int q = 0; q < NQ; q++) {
-        float m = v[q];
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        V m = v[q];
+        for (int off = 32; off > 0; off >>= 1) m = tmax(m, (V)__shfl_xor(m, off, 64));
         if (lane == 0) lds[q][wave] = m;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int q = 0; q < 2; q++) {
-            float m = 0.0f;
+            V m = V(0);
             if (q < NQ)
-                for (int w = 0; w < NT / 64; w++) m = fmaxf(m, lds[q][w]);
+                for (int w = 0; w < NT / 64; w++) m = tmax(m, lds[q][w]);
             tile_max[q * ntiles + tile] = m;
         }
     }
 }
 
-template <int NT>
-__device__ __forceinline__ void block_max_atomic(float v, float* lds_scratch, unsigned* target) {
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+template <int NT, class V, class TG>
+__device__ __forceinline__ void block_max_atomic(V v, V* lds_scratch, TG* target) {
+    for (int off = 32; off > 0; off >>= 1) v = tmax(v, (V)__shfl_xor(v, off, 64));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) lds_scratch[wave] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        float m = lds_scratch[0];
-        for (int w = 1; w < NT / 64; w++) m = fmaxf(m, lds_scratch[w]);
+        V m = lds_scratch[0];
+        for (int w = 1; w < NT / 64; w++) m = tmax(m, lds_scratch[w]);
         slot_max(target, m);
     }
 }

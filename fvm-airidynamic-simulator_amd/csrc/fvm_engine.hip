@@ -230,7 +230,7 @@ static size_t plane_index(int s, int f) {
 
 // Plane layout of one case inside its allocation.
 struct CaseLayout {
-    size_t n, plane, sets_off, k_off, ct_off, sdf_off, slots_off, t_off, steps_off, tmax_off, total;
+    size_t n, plane, sets_off, k_off, ct_off, sdf_off, slots_off, t_off, steps_off, tmax_off, tmax64_off, total;
 };
 
 static CaseLayout layout_for(const fvm_params& p, bool fp64, bool kahan, int nsets) {
@@ -256,6 +256,8 @@ static CaseLayout layout_for(const fvm_params& p, bool fp64, bool kahan, int nse
     // tile_y >= 4, so ceil(nx/32) * ceil(ny/4) bounds the tile count
     L.tmax_off = off;
     off += round256((size_t)((p.nx + 31) / 32) * ((p.ny + 3) / 4) * 2 * sizeof(float));
+    L.tmax64_off = off;
+    off += round256((size_t)((p.nx + 31) / 32) * ((p.ny + 3) / 4) * 2 * sizeof(double));
     L.total = off;
     return L;
 }
@@ -391,12 +393,18 @@ int Engine::upload_desc(HostCase& hc) {
         d->b = b;
         d->tiles_x = hc.tiles_x;
         d->tiles_y = hc.tiles_y;
+        d->dt64 = dt64() ? 1 : 0;
+        d->slots64 = (double*)(base + L.slots_off + kSlots64Off);
+        d->tile_max64 = (double*)(base + L.tmax64_off);
     } else {
         CaseDesc<float>* d = (CaseDesc<float>*)hd;
         fill_consts(d->c, hc.p);
         d->b = b;
         d->tiles_x = hc.tiles_x;
         d->tiles_y = hc.tiles_y;
+        d->dt64 = dt64() ? 1 : 0;
+        d->slots64 = (double*)(base + L.slots_off + kSlots64Off);
+        d->tile_max64 = (double*)(base + L.tmax64_off);
     }
     FVM_HIP(hipMemcpyAsync((char*)d_cases_ + (size_t)hc.id * desc_size_, hd, desc_size_, hipMemcpyHostToDevice, stream_));
     FVM_HIP(hipStreamSynchronize(stream_));
@@ -679,9 +687,10 @@ int Engine::mark_and_lists(HostCase& hc, int fixup) {
     launch_tiled(fp64_, tx_, ty_, kc);
     launch_ghost_prep(fp64_, stream_, d_cases_, d_g, (int)gl.size());
     if (!gl.empty())
-        launch_ghost(fp64_, false, dim3((unsigned)((gl.size() + 255) / 256)), stream_, d_cases_, d_g, d_cnt + 1, r, r);
+        launch_ghost(fp64_, dt64(), false, dim3((unsigned)((gl.size() + 255) / 256)), stream_, d_cases_, d_g, d_cnt + 1, r, r);
     // CFL maxima of the reset state into the slot the next step reads
     launch_zero_u32(stream_, (unsigned*)((char*)hc.mem + L.slots_off) + phase_ * 2, 2);
+    launch_zero_u32(stream_, (unsigned*)((char*)hc.mem + L.slots_off + kSlots64Off) + phase_ * 4, 4);
     kc.op = OP_MAXIMA;
     launch_tiled(fp64_, tx_, ty_, kc);
     FVM_HIP(hipGetLastError());
@@ -939,7 +948,7 @@ int Engine::enqueue_step(int ph) {
         launch_tiled(fp64_, tx_, ty_, kc);
         rc = timed("diffuse_half2", rc);
     }
-    launch_ghost(fp64_, true, dim3(grid_ghosts_ + grid_reduce_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g2,
+    launch_ghost(fp64_, dt64(), true, dim3(grid_ghosts_ + grid_reduce_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g2,
                  v_g2, grid_ghosts_, d_case_list_, d_counts_ + 3, single_case_);
     rc = timed("ghost2", rc);
     if (rc) return rc;
@@ -1119,6 +1128,7 @@ int Engine::write_state(int id, const void* host) {
     WorkItem* const d_t = scr_tiles_;
     int* const d_cnt = scr_cnt_;
     FVM_HIP(hipMemsetAsync((char*)hc->mem + L.slots_off + phase_ * 8, 0, 8, stream_));
+    FVM_HIP(hipMemsetAsync((char*)hc->mem + L.slots_off + kSlots64Off + phase_ * 16, 0, 16, stream_));
     if (kahan_) FVM_HIP(hipMemsetAsync((char*)hc->mem + L.k_off, 0, round256(L.n * 4) * 5, stream_));
     Roles r{cur, cur, -1, -1, -1, phase_, phase_, 0, 0};
     KernelCall kc{};
@@ -1168,19 +1178,29 @@ int Engine::queue_maxima(int id) {
     HostCase* hc = get(id);
     if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
     CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
-    FVM_HIP(hipMemcpyAsync(h_pinned_, (char*)hc->mem + L.slots_off + phase_ * 8, 8, hipMemcpyDeviceToHost, stream_));
+    if (dt64())
+        FVM_HIP(hipMemcpyAsync(h_pinned_, (char*)hc->mem + L.slots_off + kSlots64Off + phase_ * 16, 16,
+                               hipMemcpyDeviceToHost, stream_));
+    else
+        FVM_HIP(hipMemcpyAsync(h_pinned_, (char*)hc->mem + L.slots_off + phase_ * 8, 8, hipMemcpyDeviceToHost, stream_));
     return FVM_OK;
 }
 
-int Engine::read_maxima(int id, float* max_speed, float* max_nu) {
+int Engine::read_maxima(int id, double* max_speed, double* max_nu) {
     int rc = queue_maxima(id);
     if (rc) return rc;
     FVM_HIP(hipStreamSynchronize(stream_));
-    unsigned bits[2];
-    std::memcpy(bits, h_pinned_, 8);
-    float a, b;
-    std::memcpy(&a, &bits[0], 4);
-    std::memcpy(&b, &bits[1], 4);
+    double a, b;
+    if (dt64()) {
+        std::memcpy(&a, h_pinned_, 8);
+        std::memcpy(&b, (char*)h_pinned_ + 8, 8);
+    } else {
+        float fa, fb;
+        std::memcpy(&fa, h_pinned_, 4);
+        std::memcpy(&fb, (char*)h_pinned_ + 4, 4);
+        a = fa;
+        b = fb;
+    }
     if (max_speed) *max_speed = a;
     if (max_nu) *max_nu = b;
     return FVM_OK;
@@ -1293,6 +1313,10 @@ int Engine::save_case(int id, const char* path) {
     FVM_HIP(hipMemcpy(bits, base + L.slots_off + phase_ * 8, 8, hipMemcpyDeviceToHost));
     h.max_speed_bits = bits[0];
     h.max_nu_bits = bits[1];
+    double m64[2];
+    FVM_HIP(hipMemcpy(m64, base + L.slots_off + kSlots64Off + phase_ * 16, 16, hipMemcpyDeviceToHost));
+    h.max_speed64 = m64[0];
+    h.max_nu64 = m64[1];
     File out(std::fopen(path, "wb"));
     if (!out.f) return set_error(FVM_ERR_INVALID_ARGUMENT, std::string("cannot create snapshot ") + path);
     bool ok = std::fwrite(&h, sizeof(h), 1, out.f) == 1 && std::fwrite(&hc->p, sizeof(fvm_params), 1, out.f) == 1;
@@ -1339,6 +1363,8 @@ int Engine::load_case(int id, const char* path) {
     }
     long long steps = h.steps;
     unsigned bits[2] = {h.max_speed_bits, h.max_nu_bits};
+    double m64[2] = {h.max_speed64, h.max_nu64};
+    FVM_HIP(hipMemcpy(base + L.slots_off + kSlots64Off + phase_ * 16, m64, 16, hipMemcpyHostToDevice));
     FVM_HIP(hipMemcpy(base + L.t_off, &h.t, 8, hipMemcpyHostToDevice));
     FVM_HIP(hipMemcpy(base + L.steps_off, &steps, 8, hipMemcpyHostToDevice));
     FVM_HIP(hipMemcpy(base + L.slots_off + phase_ * 8, bits, 8, hipMemcpyHostToDevice));

@@ -21,6 +21,10 @@ int set_error(int code, const std::string& msg);
 int read_snapshot_header(const char* path, fvm_snapshot_header* h, fvm_params* p);
 int hip_fail(hipError_t e, const char* what);
 
+// Byte offset of the FP64 CFL slots inside a case's 256-byte slot block (the
+// FP32 slots are at 0).
+constexpr size_t kSlots64Off = 64;
+
 #define FVM_HIP(call)                                  \
     do {                                               \
         hipError_t e_ = (call);                        \
@@ -109,7 +113,9 @@ public:
     int read_cell_types(int id, uint8_t* host);
     int read_sdf(int id, float* host);
     int read_time(int id, double* t, long long* steps);
-    int read_maxima(int id, float* max_speed, float* max_nu);
+    // CFL maxima the next step's dt comes from (FP64 maxima for dt64 engines,
+    // else the FP32 ones widened)
+    int read_maxima(int id, double* max_speed, double* max_nu);
     int queue_maxima(int id);
     int field(int id, int field, float* dev_out);
     int stat(int id, int mode, float* out);
@@ -127,6 +133,8 @@ public:
     int reserve_cases(int n);
     bool fp64() const { return fp64_; }
     bool kahan() const { return kahan_; }
+    // CFL maxima and dt formula in FP64 (FP64 state and compensated variant)
+    bool dt64() const { return fp64_ || kahan_; }
     hipStream_t stream() const { return stream_; }
 
     void enable_timing(bool on) { timing_ = on; }

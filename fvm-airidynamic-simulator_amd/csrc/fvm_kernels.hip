@@ -17,6 +17,7 @@
 #include "fvm_kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fvm {
 
@@ -416,7 +417,10 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
     // five flux components + the interface velocity of the -p div(v) source
     // (mass flux over the mean clamped density of the two cells, solver.cu:1787-1794)
     __shared__ T s_f[6][FMAX];
-    __shared__ float s_red[1][NT / 64];
+    // CFL maxima: FP64 for the FP64 state and the compensated variant
+    constexpr bool DT64 = KAHAN || sizeof(T) == 8;
+    using MT = typename std::conditional<DT64, double, float>::type;
+    __shared__ MT s_red[1][NT / 64];
 
     const int n_items = *count;
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
@@ -428,12 +432,15 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
         const Roles r = visc ? rv : ri;
         const int nx = k.nx, ny = k.ny;
         const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
-        const float dtf = dt_from_slot(k, cd.b.slots, r.slot_r);
-        const T dt = (T)dtf;
+        const double dtd = dt_from_slot<DT64>(cd, r.slot_r);
+        const T dt = (T)dtd;
 
         if (it.index == 0 && threadIdx.x == 0) {
-            if (r.zero_slot) { cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u; }
-            if (r.count_step) { cd.b.t[0] += (double)dtf; cd.b.steps[0] += 1; }
+            if (r.zero_slot) {
+                cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
+                if constexpr (DT64) { cd.slots64[r.slot_w * 2] = 0.0; cd.slots64[r.slot_w * 2 + 1] = 0.0; }
+            }
+            if (r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
         }
 
         const Plane<T> in0(cd.b.U[r.in][0]), in1(cd.b.U[r.in][1]), in2(cd.b.U[r.in][2]), in3(cd.b.U[r.in][3]),
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
         }
         __syncthreads();
 
-        float vmax_local = 0.0f;
+        MT vmax_local = MT(0);
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
             int n = threadIdx.x + q * NT;
@@ -628,7 +635,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
                     continue;  // filled by k_ghost
                 }
                 bool touched = bc_pointwise(k, gi, gj, c, out);
-                if (MODE == 1) vmax_local = fmaxf(vmax_local, (float)wave_speed(k, out));
+                if (MODE == 1) vmax_local = tmax(vmax_local, wave_speed_as<MT>(k, out));
                 if (KAHAN && MODE == 1)
                     for (int f = 0; f < 5; f++) Plane<float>(cd.b.K[f]).put(g, touched ? 0.0f : (float)carry[f]);
             }
@@ -636,8 +643,9 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
             for (int f = 0; f < 5; f++) Plane<T>(cd.b.U[r.out][f]).put(g, out[f]);
         }
         if (MODE == 1 && !visc) {
-            const float vm[1] = {vmax_local};
-            block_max_store<NT, 1>(vm, s_red, cd.b.tile_max, cd.tiles_x * cd.tiles_y, it.index);
+            const MT vm[1] = {vmax_local};
+            if constexpr (DT64) block_max_store<NT, 1>(vm, s_red, cd.tile_max64, cd.tiles_x * cd.tiles_y, it.index);
+            else block_max_store<NT, 1>(vm, s_red, cd.b.tile_max, cd.tiles_x * cd.tiles_y, it.index);
         }
         __syncthreads();
     }
@@ -703,15 +711,17 @@ __global__ __launch_bounds__(NT) void k_ghost_prep(const CaseDesc<T>* __restrict
 // Wave-level maximum into a case's CFL slot counter: when every lane of the
 // wave works on the same case, one atomic per wave; otherwise one per lane.
 // Called by whole waves.
-__device__ __forceinline__ void wave_slot_max(unsigned* target, bool active, int case_id, float v) {
+// FP32 maxima go to an unsigned bit-pattern slot, FP64 maxima to a double slot.
+template <class V, class TG>
+__device__ __forceinline__ void wave_slot_max(TG* target, bool active, int case_id, V v) {
     const int c0 = __shfl(case_id, 0, 64);
     const bool uniform = __all(!active || case_id == c0);
     if (uniform) {
-        float m = active ? v : 0.0f;
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-        if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(target, __float_as_uint(m));
-    } else if (active && v > 0.0f) {
-        atomicMax(target, __float_as_uint(v));
+        V m = active ? v : V(0);
+        for (int off = 32; off > 0; off >>= 1) m = tmax(m, (V)__shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0 && m > V(0)) atomic_max_bits(target, m);
+    } else if (active && v > V(0)) {
+        atomic_max_bits(target, v);
     }
 }
 
@@ -720,50 +730,59 @@ __device__ __forceinline__ void wave_slot_max(unsigned* target, bool active, int
 // into each case's CFL slot, one item (case, chunk of REDUCE_CHUNK tiles) per
 // workgroup pass and one atomic per item and quantity.  Each thread issues its
 // loads of a chunk together, so a chunk costs one memory latency.
-template <class T>
+// One chunk of one case's per-tile maxima (V = float or double) into its slot.
+template <class V, class TG>
+__device__ __forceinline__ void reduce_tile_chunk(const V* tile_max, int nt, int chunk, TG* slot, V (*s_red)[NT / 64]) {
+    constexpr int PER = REDUCE_CHUNK / NT;
+    const Plane<V> tm(tile_max);
+    const int t0 = chunk * REDUCE_CHUNK + threadIdx.x;
+    V a0[PER], a1[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int t = t0 + q * NT;
+        a0[q] = t < nt ? tm[(unsigned)t] : V(0);
+        a1[q] = t < nt ? tm[(unsigned)(nt + t)] : V(0);
+    }
+    V m0 = V(0), m1 = V(0);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        m0 = tmax(m0, a0[q]);
+        m1 = tmax(m1, a1[q]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = tmax(m0, (V)__shfl_xor(m0, off, 64));
+        m1 = tmax(m1, (V)__shfl_xor(m1, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { s_red[0][wave] = m0; s_red[1][wave] = m1; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        V a = s_red[0][0], c = s_red[1][0];
+        for (int w = 1; w < NT / 64; w++) { a = tmax(a, s_red[0][w]); c = tmax(c, s_red[1][w]); }
+        if (a > V(0)) atomic_max_bits(&slot[0], a);
+        if (c > V(0)) atomic_max_bits(&slot[1], c);
+    }
+    __syncthreads();
+}
+
+// DT64: the engine keeps FP64 maxima (every case of an engine does the same).
+template <class T, bool DT64>
 __device__ void reduce_tile_maxima(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ case_list,
                                    int nitems, int rblock, int nrblocks, Roles ri, Roles rv) {
-    __shared__ float s_red[2][NT / 64];
-    constexpr int PER = REDUCE_CHUNK / NT;
+    using MT = typename std::conditional<DT64, double, float>::type;
+    __shared__ MT s_red[2][NT / 64];
     for (int ci = rblock; ci < nitems; ci += nrblocks) {
         const WorkItem it = case_list[ci];
         const CaseDesc<T>& cd = cases[it.case_id];
         const Roles r = cd.c.visc ? rv : ri;
         const int nt = cd.tiles_x * cd.tiles_y;
-        const Plane<float> tm(cd.b.tile_max);
-        const int t0 = it.index * REDUCE_CHUNK + threadIdx.x;
-        float a0[PER], a1[PER];
-#pragma unroll
-        for (int q = 0; q < PER; q++) {
-            const int t = t0 + q * NT;
-            a0[q] = t < nt ? tm[(unsigned)t] : 0.0f;
-            a1[q] = t < nt ? tm[(unsigned)(nt + t)] : 0.0f;
-        }
-        float m0 = 0.0f, m1 = 0.0f;
-#pragma unroll
-        for (int q = 0; q < PER; q++) {
-            m0 = fmaxf(m0, a0[q]);
-            m1 = fmaxf(m1, a1[q]);
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            m0 = fmaxf(m0, __shfl_xor(m0, off, 64));
-            m1 = fmaxf(m1, __shfl_xor(m1, off, 64));
-        }
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) { s_red[0][wave] = m0; s_red[1][wave] = m1; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float a = s_red[0][0], c = s_red[1][0];
-            for (int w = 1; w < NT / 64; w++) { a = fmaxf(a, s_red[0][w]); c = fmaxf(c, s_red[1][w]); }
-            if (a > 0.0f) atomicMax(&cd.b.slots[r.slot_w * 2], __float_as_uint(a));
-            if (c > 0.0f) atomicMax(&cd.b.slots[r.slot_w * 2 + 1], __float_as_uint(c));
-        }
-        __syncthreads();
+        if constexpr (DT64) reduce_tile_chunk<double>(cd.tile_max64, nt, it.index, &cd.slots64[r.slot_w * 2], s_red);
+        else reduce_tile_chunk<float>(cd.b.tile_max, nt, it.index, &cd.b.slots[r.slot_w * 2], s_red);
     }
 }
 
 // One ghost cell per lane: fill, store, and (FINAL) its CFL maxima.
-template <class T, bool FINAL>
+template <class T, bool FINAL, bool DT64>
 __device__ __forceinline__ void ghost_item(const CaseDesc<T>& cd, const WorkItem it, bool active, Roles ri, Roles rv) {
     const CaseConsts<T>& k = cd.c;
     const Roles r = k.visc ? rv : ri;
@@ -779,24 +798,31 @@ __device__ __forceinline__ void ghost_item(const CaseDesc<T>& cd, const WorkItem
     if (active)
         for (int f = 0; f < 5; f++) Plane<T>(b.U[r.out][f]).put(idx, o[f]);
     if (FINAL) {
-        const float s = active ? (float)wave_speed(k, o) : 0.0f;
         T u_, v_, p_, t_;
         primitives(k, pv[0], pv[1], pv[2], pv[3], pv[4], u_, v_, p_, t_);
-        const float nu_c = (float)(sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f)));
-        const float nu = (active && k.visc) ? nu_c : 0.0f;
+        const T nu_c = sutherland(k, t_) * Num<T>::rcp(o[0] + T(1e-10f));
         // every lane takes part; a lane of another case keeps its own counter
-        wave_slot_max(&b.slots[r.slot_w * 2], active, it.case_id, s);
-        wave_slot_max(&b.slots[r.slot_w * 2 + 1], active && k.visc, it.case_id, nu);
+        if constexpr (DT64) {
+            const double s = active ? wave_speed_as<double>(k, o) : 0.0;
+            const double nu = (active && k.visc) ? (double)nu_c : 0.0;
+            wave_slot_max(&cd.slots64[r.slot_w * 2], active, it.case_id, s);
+            wave_slot_max(&cd.slots64[r.slot_w * 2 + 1], active && k.visc, it.case_id, nu);
+        } else {
+            const float s = active ? wave_speed_as<float>(k, o) : 0.0f;
+            const float nu = (active && k.visc) ? (float)nu_c : 0.0f;
+            wave_slot_max(&b.slots[r.slot_w * 2], active, it.case_id, s);
+            wave_slot_max(&b.slots[r.slot_w * 2 + 1], active && k.visc, it.case_id, nu);
+        }
     }
 }
 
-template <class T, bool FINAL>
+template <class T, bool FINAL, bool DT64>
 __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ items,
                                               const int* __restrict__ count, Roles ri, Roles rv, int gblocks,
                                               const WorkItem* __restrict__ case_list, const int* __restrict__ ncases,
                                               int single_case) {
     if (FINAL && (int)blockIdx.x >= gblocks) {
-        reduce_tile_maxima(cases, case_list, *ncases, (int)blockIdx.x - gblocks, (int)gridDim.x - gblocks, ri, rv);
+        reduce_tile_maxima<T, DT64>(cases, case_list, *ncases, (int)blockIdx.x - gblocks, (int)gridDim.x - gblocks, ri, rv);
         return;
     }
     const int n_items = *count;
@@ -806,7 +832,7 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
         const bool active = w < n_items;
         if (single_case >= 0) {
             // one case: its list is (case, 0 .. n-1), no item to load
-            ghost_item<T, FINAL>(cases[single_case], WorkItem{single_case, active ? w : base}, active, ri, rv);
+            ghost_item<T, FINAL, DT64>(cases[single_case], WorkItem{single_case, active ? w : base}, active, ri, rv);
             continue;
         }
         const WorkItem it = items[active ? w : base];
@@ -815,8 +841,8 @@ __global__ __launch_bounds__(NT) void k_ghost(const CaseDesc<T>* __restrict__ ca
         // every buffer pointer into a vector load that the field load then
         // waits for.
         const int c0 = __builtin_amdgcn_readfirstlane(it.case_id);
-        if (__all(it.case_id == c0)) ghost_item<T, FINAL>(cases[c0], it, active, ri, rv);
-        else ghost_item<T, FINAL>(cases[it.case_id], it, active, ri, rv);
+        if (__all(it.case_id == c0)) ghost_item<T, FINAL, DT64>(cases[c0], it, active, ri, rv);
+        else ghost_item<T, FINAL, DT64>(cases[it.case_id], it, active, ri, rv);
     }
 }
 #pragma clang fp contract(on)
@@ -847,7 +873,8 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
     __shared__ T s_u[SY * SX], s_v[SY * SX], s_T[SY * SX];
     __shared__ uint8_t s_ct[SY * SX];
     __shared__ T s_txx[QY * QX], s_tyy[QY * QX], s_txy[QY * QX], s_qx[QY * QX], s_qy[QY * QX];
-    __shared__ float s_red[2][NT / 64];
+    // CFL maxima in the state's precision (FP64 state: dt64)
+    __shared__ T s_red[2][NT / 64];
 
     const int n_items = *count;
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
@@ -857,11 +884,14 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
         const CaseConsts<T>& k = cd.c;
         const int nx = k.nx, ny = k.ny;
         const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
-        const float dtf = dt_from_slot(k, cd.b.slots, r.slot_r);
-        const T hdt = (T)(dtf * 0.5f);
+        const double dtd = dt_from_slot<(sizeof(T) == 8)>(cd, r.slot_r);
+        const T hdt = (T)(dtd * 0.5);
         if (it.index == 0 && threadIdx.x == 0) {
-            if (r.zero_slot) { cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u; }
-            if (r.count_step) { cd.b.t[0] += (double)dtf; cd.b.steps[0] += 1; }
+            if (r.zero_slot) {
+                cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
+                if constexpr (sizeof(T) == 8) { cd.slots64[r.slot_w * 2] = 0.0; cd.slots64[r.slot_w * 2 + 1] = 0.0; }
+            }
+            if (r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
         }
         const Plane<T> in[5] = {Plane<T>(cd.b.U[r.in][0]), Plane<T>(cd.b.U[r.in][1]), Plane<T>(cd.b.U[r.in][2]),
                                 Plane<T>(cd.b.U[r.in][3]), Plane<T>(cd.b.U[r.in][4])};
@@ -915,7 +945,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
         }
         __syncthreads();
 
-        float smax = 0.0f, numax = 0.0f;
+        T smax = T(0), numax = T(0);
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
             int n = threadIdx.x + q * NT;
@@ -955,16 +985,17 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
             if (FINAL) {
                 if (c == CT_GHOST) continue;  // filled by k_ghost
                 bc_pointwise(k, gi, gj, c, v);
-                smax = fmaxf(smax, (float)wave_speed(k, v));
+                smax = tmax(smax, wave_speed_as<T>(k, v));
                 T mu = (c == CT_SOLID) ? T(0) : sutherland(k, Tin);
-                numax = fmaxf(numax, (float)(mu * Num<T>::rcp(v[0] + T(1e-10f))));
+                numax = tmax(numax, mu * Num<T>::rcp(v[0] + T(1e-10f)));
             }
             // the first half step leaves rho alone: set D's density plane is set A's
             for (int f = FINAL ? 0 : 1; f < 5; f++) outp[f].put(g, v[f]);
         }
         if (FINAL) {
-            const float vm[2] = {smax, numax};
-            block_max_store<NT, 2>(vm, s_red, cd.b.tile_max, cd.tiles_x * cd.tiles_y, it.index);
+            const T vm[2] = {smax, numax};
+            if constexpr (sizeof(T) == 8) block_max_store<NT, 2>(vm, s_red, cd.tile_max64, cd.tiles_x * cd.tiles_y, it.index);
+            else block_max_store<NT, 2>(vm, s_red, cd.b.tile_max, cd.tiles_x * cd.tiles_y, it.index);
         }
         __syncthreads();
     }
@@ -975,7 +1006,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
 template <class T, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_maxima(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
                                                const int* __restrict__ count, Roles r) {
-    __shared__ float s_red[2][NT / 64];
+    __shared__ double s_red[2][NT / 64];
     const int n_items = *count;
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
         const WorkItem it = work[w];
@@ -983,7 +1014,8 @@ __global__ __launch_bounds__(NT) void k_maxima(const CaseDesc<T>* __restrict__ c
         const CaseDesc<T>& cd = cases[it.case_id];
         const CaseConsts<T>& k = cd.c;
         const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
-        float smax = 0.0f, numax = 0.0f;
+        // FP32 maxima (exact as doubles) or FP64 ones for dt64 cases
+        double smax = 0.0, numax = 0.0;
         for (int q = threadIdx.x; q < TX * TY; q += NT) {
             int gi = i0 + q % TX, gj = j0 + q / TX;
             if (gi >= k.nx || gj >= k.ny) continue;
@@ -991,12 +1023,24 @@ __global__ __launch_bounds__(NT) void k_maxima(const CaseDesc<T>* __restrict__ c
             T v[5];
             for (int f = 0; f < 5; f++) v[f] = ((const T*)cd.b.U[r.in][f])[g];
             T tt;
-            smax = fmaxf(smax, (float)wave_speed(k, v, &tt));
+            const T ws = wave_speed(k, v, &tt);
             T mu = (cd.b.ct[g] == CT_SOLID) ? T(0) : sutherland(k, tt);
-            numax = fmaxf(numax, (float)(mu * Num<T>::rcp(v[0] + T(1e-10f))));
+            const T nu = mu * Num<T>::rcp(v[0] + T(1e-10f));
+            if (cd.dt64) {
+                smax = fmax(smax, wave_speed_as<double>(k, v));
+                numax = fmax(numax, (double)nu);
+            } else {
+                smax = fmax(smax, (double)(float)ws);
+                numax = fmax(numax, (double)(float)nu);
+            }
         }
-        block_max_atomic<NT>(smax, s_red[0], &cd.b.slots[r.slot_w * 2]);
-        block_max_atomic<NT>(numax, s_red[1], &cd.b.slots[r.slot_w * 2 + 1]);
+        if (cd.dt64) {
+            block_max_atomic<NT>(smax, s_red[0], &cd.slots64[r.slot_w * 2]);
+            block_max_atomic<NT>(numax, s_red[1], &cd.slots64[r.slot_w * 2 + 1]);
+        } else {
+            block_max_atomic<NT>((float)smax, (float*)s_red[0], &cd.b.slots[r.slot_w * 2]);
+            block_max_atomic<NT>((float)numax, (float*)s_red[1], &cd.b.slots[r.slot_w * 2 + 1]);
+        }
         __syncthreads();
     }
 }
@@ -1152,8 +1196,11 @@ __global__ __launch_bounds__(64) void k_cons_final(const double* partial, int nb
 // ---------------------------------------------------------------------------
 template <class T, int TX, int TY>
 struct Launch {
-    static void stage1(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv) {
-        k_stage<T, TX, TY, 0, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+    // Stage 1 of the compensated variant is the KAHAN instantiation only for
+    // its FP64 dt (the compensation itself is in the blend of stage 2).
+    static void stage1(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv, bool kahan) {
+        if (kahan && sizeof(T) == 4) k_stage<T, TX, TY, 0, true><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+        else k_stage<T, TX, TY, 0, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
     }
     static void stage2(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv, bool kahan) {
         if (kahan) k_stage<T, TX, TY, 1, true><<<g, NT, 0, s>>>(c, w, n, ri, rv);
@@ -1182,7 +1229,7 @@ static bool dispatch_tile(int tx, int ty, const KernelCall& kc);
 #define FVM_TRY(A, B)                                                                              \
     if (tx == A && ty == B) {                                                                      \
         switch (kc.op) {                                                                           \
-        case OP_STAGE1: Launch<T, A, B>::stage1(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv); break; \
+        case OP_STAGE1: Launch<T, A, B>::stage1(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan); break; \
         case OP_STAGE2: Launch<T, A, B>::stage2(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan); break; \
         case OP_DIFFUSE: Launch<T, A, B>::diffuse(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.rv, false); break; \
         case OP_DIFFUSE_FINAL: Launch<T, A, B>::diffuse(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.rv, true); break; \
@@ -1224,16 +1271,21 @@ bool tile_supported(int tx, int ty, bool fp64) {
     return false;
 }
 
-void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
+void launch_ghost(bool fp64, bool dt64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
                   const int* n, Roles ri, Roles rv, int gblocks, const WorkItem* case_list, const int* ncases,
                   int single_case) {
     if (!final_) gblocks = (int)grid.x;
+    const CaseDesc<double>* c64 = (const CaseDesc<double>*)c;
+    const CaseDesc<float>* c32 = (const CaseDesc<float>*)c;
     if (fp64) {
-        if (final_) k_ghost<double, true><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv, gblocks, case_list, ncases, single_case);
-        else k_ghost<double, false><<<grid, NT, 0, s>>>((const CaseDesc<double>*)c, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        if (final_) k_ghost<double, true, true><<<grid, NT, 0, s>>>(c64, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        else k_ghost<double, false, true><<<grid, NT, 0, s>>>(c64, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+    } else if (dt64) {
+        if (final_) k_ghost<float, true, true><<<grid, NT, 0, s>>>(c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        else k_ghost<float, false, true><<<grid, NT, 0, s>>>(c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
     } else {
-        if (final_) k_ghost<float, true><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv, gblocks, case_list, ncases, single_case);
-        else k_ghost<float, false><<<grid, NT, 0, s>>>((const CaseDesc<float>*)c, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        if (final_) k_ghost<float, true, false><<<grid, NT, 0, s>>>(c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        else k_ghost<float, false, false><<<grid, NT, 0, s>>>(c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
     }
 }
 

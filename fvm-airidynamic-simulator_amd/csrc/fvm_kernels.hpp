@@ -29,7 +29,8 @@ constexpr int REDUCE_CHUNK = 1024;
 // which also takes the ghost cells' CFL maxima and, in workgroups
 // [gblocks, grid.x), reduces the per-tile maxima of the items in case_list.
 // single_case >= 0: the list is (single_case, 0 .. n-1) and is not read.
-void launch_ghost(bool fp64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
+// dt64: the engine keeps its CFL maxima in FP64 (FP64 state, compensated variant).
+void launch_ghost(bool fp64, bool dt64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
                   const int* n, Roles ri, Roles rv, int gblocks = 0, const WorkItem* case_list = nullptr,
                   const int* ncases = nullptr, int single_case = -1);
 void launch_ghost_prep(bool fp64, hipStream_t s, const void* c, const WorkItem* items, int n);

@@ -36,7 +36,9 @@ class SnapshotHeader(ctypes.Structure):
         ("steps", ctypes.c_int64),
         ("max_speed_bits", ctypes.c_uint32),
         ("max_nu_bits", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint8 * 192),
+        ("max_speed64", ctypes.c_double),
+        ("max_nu64", ctypes.c_double),
+        ("reserved", ctypes.c_uint8 * 176),
     ]
 
 

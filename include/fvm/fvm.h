@@ -233,7 +233,9 @@ typedef struct fvm_snapshot_header {
     int64_t steps;              /* device step count */
     uint32_t max_speed_bits;    /* CFL maxima of the state (float bit patterns) */
     uint32_t max_nu_bits;
-    uint8_t reserved[192];
+    double max_speed64;         /* the same in FP64 (FP64 and compensated variants) */
+    double max_nu64;
+    uint8_t reserved[176];
 } fvm_snapshot_header;
 
 /* Writes the current state of the solver to `path`. */

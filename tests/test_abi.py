@@ -89,7 +89,7 @@ def test_create_without_gpu_fails_cleanly():
 def test_snapshot_header_layout_matches_c():
     from fvm_amd.snapshot import SnapshotHeader as H
     fields = ["version", "header_bytes", "abi_version", "params_bytes", "nx", "ny", "elem_bytes", "kahan", "t",
-              "steps", "max_speed_bits", "max_nu_bits", "reserved"]
+              "steps", "max_speed_bits", "max_nu_bits", "max_speed64", "max_nu64", "reserved"]
     src = "#include <stdio.h>\n#include <stddef.h>\n#include \"fvm/fvm.h\"\nint main(void) {\n"
     src += '  printf("%zu", sizeof(fvm_snapshot_header));\n'
     for f in fields:
