@@ -6,7 +6,7 @@ OUT="${1:-$HERE/../lib}"
 mkdir -p "$OUT" "$HERE/../build"
 FLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Wall -Xclang -target-feature -Xclang -packed-fp32-ops"
 pids=()
-for f in fvm_kernels fvm_engine fvm_api; do
+for f in fvm_kernels fvm_engine fvm_api fvm_noalloc; do
   hipcc $FLAGS -c "$HERE/$f.hip" -o "$HERE/../build/$f.o" 2> "$HERE/../build/$f.log" &
   pids+=($!)
 done
@@ -14,9 +14,9 @@ rc=0
 for i in "${!pids[@]}"; do
   if ! wait "${pids[$i]}"; then rc=1; fi
 done
-for f in fvm_kernels fvm_engine fvm_api; do grep -v "packed-fp32-ops" "$HERE/../build/$f.log" | grep -v "^$" || true; done
+for f in fvm_kernels fvm_engine fvm_api fvm_noalloc; do grep -v "packed-fp32-ops" "$HERE/../build/$f.log" | grep -v "^$" || true; done
 [ $rc -eq 0 ] || { echo "compile failed"; exit 1; }
-hipcc --offload-arch=gfx950 -shared -fPIC -o "$OUT/libfvm_amd.so.tmp" "$HERE/../build/fvm_kernels.o" "$HERE/../build/fvm_engine.o" "$HERE/../build/fvm_api.o"
+hipcc --offload-arch=gfx950 -shared -fPIC -o "$OUT/libfvm_amd.so.tmp" "$HERE/../build/fvm_kernels.o" "$HERE/../build/fvm_engine.o" "$HERE/../build/fvm_api.o" "$HERE/../build/fvm_noalloc.o"
 mv "$OUT/libfvm_amd.so.tmp" "$OUT/libfvm_amd.so"
 echo "built $OUT/libfvm_amd.so"
 # KernelBenchmark counterpart (host C++ over the C ABI)

@@ -128,7 +128,7 @@ int Engine::init_stream() {
     }
     if (!d_stat_) FVM_HIP(hipMalloc((void**)&d_stat_, 16 * sizeof(unsigned)));
     if (!scr_cnt_) FVM_HIP(hipMalloc((void**)&scr_cnt_, 4 * sizeof(int)));
-    if (!d_cons_) FVM_HIP(hipMalloc((void**)&d_cons_, (1024 * 4 + 8) * sizeof(double)));
+    if (!d_cons_) FVM_HIP(hipMalloc((void**)&d_cons_, (kConsBlocks * 4 + 8) * sizeof(double)));
     if (!h_pinned_) FVM_HIP(hipHostMalloc((void**)&h_pinned_, 16 * sizeof(float)));
     for (auto& ts : tset_)
         for (auto& ev : ts.ev)
@@ -1240,7 +1240,11 @@ int Engine::stat(int id, int mode, float* out) {
 int Engine::conservation(int id, double out[4]) {
     HostCase* hc = get(id);
     if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
-    launch_cons(fp64_, stream_, d_cases_, id, current_set(*hc), d_cons_ + 8, 1024, d_cons_);
+    CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
+    const int cur = current_set(*hc);
+    const void* planes[4];
+    for (int f = 0; f < 4; f++) planes[f] = (const char*)hc->mem + L.sets_off + L.plane * plane_index(cur, f);
+    launch_cons(fp64_, stream_, planes, L.n, d_cons_ + 8, kConsBlocks, d_cons_);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(stream_));
     FVM_HIP(hipMemcpy(out, d_cons_, 4 * sizeof(double), hipMemcpyDeviceToHost));
@@ -1250,7 +1254,7 @@ int Engine::conservation(int id, double out[4]) {
 int Engine::forces(int id, double out[3]) {
     HostCase* hc = get(id);
     if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
-    launch_forces(fp64_, stream_, d_cases_, id, current_set(*hc), d_cons_ + 8, 1024, d_cons_);
+    launch_forces(fp64_, stream_, d_cases_, id, current_set(*hc), d_cons_ + 8, kConsBlocks, d_cons_);
     FVM_HIP(hipGetLastError());
     FVM_HIP(hipStreamSynchronize(stream_));
     double r[4];

@@ -1106,16 +1106,18 @@ __global__ __launch_bounds__(NT) void k_stat(const CaseDesc<T>* __restrict__ cas
     block_max_atomic<NT>(m, s_red, target);
 }
 
-// Domain totals of rho, rho_u, rho_v, E in FP64: per-workgroup partials, then
-// one workgroup sums them in a fixed order (deterministic).
+// Domain totals of four planes (rho, rho_u, rho_v, E) of n cells in FP64:
+// per-workgroup partials, then one workgroup sums them in a fixed order
+// (deterministic for a given number of workgroups).
 template <class T>
-__global__ __launch_bounds__(NT) void k_cons_partial(const CaseDesc<T>* __restrict__ cases, int case_id, int set, double* partial) {
+__global__ __launch_bounds__(NT) void k_cons_partial(const T* __restrict__ p0, const T* __restrict__ p1,
+                                                     const T* __restrict__ p2, const T* __restrict__ p3, size_t n,
+                                                     double* partial) {
     __shared__ double s[4][NT];
-    const CaseDesc<T>& cd = cases[case_id];
-    size_t n = (size_t)cd.c.nx * cd.c.ny;
+    const T* const pl[4] = {p0, p1, p2, p3};
     double acc[4] = {0, 0, 0, 0};
     for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT)
-        for (int f = 0; f < 4; f++) acc[f] += (double)((const T*)cd.b.U[set][f])[i];
+        for (int f = 0; f < 4; f++) acc[f] += (double)pl[f][i];
     for (int f = 0; f < 4; f++) s[f][threadIdx.x] = acc[f];
     __syncthreads();
     for (int off = NT / 2; off > 0; off >>= 1) {
@@ -1330,9 +1332,14 @@ void launch_stat(bool fp64, hipStream_t s, const void* c, int case_id, size_t nc
     else k_stat<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, mode, target);
 }
 
-void launch_cons(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out) {
-    if (fp64) k_cons_partial<double><<<nblocks, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set, partial);
-    else k_cons_partial<float><<<nblocks, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, partial);
+void launch_cons(bool fp64, hipStream_t s, const void* const planes[4], size_t n, double* partial, int nblocks,
+                 double* out) {
+    if (fp64)
+        k_cons_partial<double><<<nblocks, NT, 0, s>>>((const double*)planes[0], (const double*)planes[1],
+                                                       (const double*)planes[2], (const double*)planes[3], n, partial);
+    else
+        k_cons_partial<float><<<nblocks, NT, 0, s>>>((const float*)planes[0], (const float*)planes[1],
+                                                      (const float*)planes[2], (const float*)planes[3], n, partial);
     k_cons_final<<<1, 64, 0, s>>>(partial, nblocks, out);
 }
 

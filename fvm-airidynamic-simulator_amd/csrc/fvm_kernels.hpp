@@ -40,7 +40,13 @@ void launch_copy_set(bool fp64, hipStream_t s, const void* c, int case_id, size_
 void launch_zero_u32(hipStream_t s, unsigned* p, int n);
 void launch_field(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int field, float* out);
 void launch_stat(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int mode, unsigned* target);
-void launch_cons(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out);
+// FP64 totals of four planes of n cells (FP32 or FP64 values): nblocks
+// partials of 4 doubles in `partial`, then their fixed-order sum into out[4].
+// Every caller takes kConsBlocks workgroups, so the totals of a state are the
+// same bits whichever entry point computes them.
+constexpr int kConsBlocks = 1024;
+void launch_cons(bool fp64, hipStream_t s, const void* const planes[4], size_t n, double* partial, int nblocks,
+                 double* out);
 void launch_forces(bool fp64, hipStream_t s, const void* c, int case_id, int set, double* partial, int nblocks, double* out);
 
 }  // namespace fvm

@@ -60,6 +60,7 @@ SIGNATURES = [
     ("fvm_set_state", _i, [_vp, _fp]),
     ("fvm_get_state_f64", _i, [_vp, _dp]),
     ("fvm_conservation_sums", _i, [_vp, _dp]),
+    ("fvm_conservation_sums_static", _i, [_vp, _i, _i, _i, _vp, _vp]),
     ("fvm_forces", _i, [_vp, _dp]),
     ("fvm_get_time", _i, [_vp, _dp, _i64p]),
     ("fvm_save_snapshot", _i, [_vp, ctypes.c_char_p]),

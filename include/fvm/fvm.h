@@ -194,6 +194,19 @@ int fvm_set_state(fvm_solver* s, const float* host5);
 int fvm_get_state_f64(fvm_solver* s, double* host5);
 /* Domain totals of (rho, rho_u, rho_v, E) summed in FP64 on the device. */
 int fvm_conservation_sums(fvm_solver* s, double out4[4]);
+/* fvm_conservation_sums without a handle and without allocating: no
+ * counterpart in the reference.  For callers that cannot call hipMalloc (code
+ * under HIP graph capture, serving loops that must not allocate).  dev_state:
+ * a state in caller device memory, SoA planes rho, rho_u, rho_v, E (a fifth
+ * plane may follow) of nx*ny values each, row-major, elem_bytes 4 (FP32) or 8
+ * (FP64).  dev_out4: 4 doubles of caller device memory, written on `stream`
+ * (a hipStream_t; NULL = the null stream); the call returns without waiting.
+ * Same kernels and workgroup count as fvm_conservation_sums, so the same bits
+ * for the same state.  Its partial sums live in one 32 KiB device array per
+ * GPU, kept until the process ends and shared by every call on that GPU:
+ * calls that overlap on one GPU race.  Rejects nx*ny > FVM_STATIC_MAX_CELLS. */
+#define FVM_STATIC_MAX_CELLS 2147483647
+int fvm_conservation_sums_static(const void* dev_state, int elem_bytes, int nx, int ny, double* dev_out4, void* stream);
 /* Pressure force on the obstacle per unit depth, (Fx, Fy), summed in FP64 over
  * the wall faces of the current state: the wall flux (0, p, 0, 0, 0) that
  * computeFluxesKernel (solver.cu:1589-1598, 1673-1682) gives a face with a
