@@ -8,6 +8,7 @@
 // SI Mach-3 viscous circle on 256x128 (the reference defaults).
 // usage: front_end_sequence [steps=20]
 #include <hip/hip_runtime_api.h>
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -77,6 +78,23 @@ int main(int argc, char* argv[]) {
         double sum = 0; float mx = h[0];
         for (size_t i = 0; i < n; i++) { sum += h[i]; mx = h[i] > mx ? h[i] : mx; }
         std::printf("sum_%s %.12e\nmax_%s %.9e\n", names[f], sum, names[f], mx);
+    }
+    // velocity arrows into a device buffer sized as the front end sizes its
+    // VBO (main.cpp:328-346, 910-934): vector density 20, arrow length 0.8 cell
+    // spacings per step, head angle 0.5 rad, head 0.3 of the shaft
+    {
+        const int step = 20;
+        const int max_vertices = ((params.nx + step - 1) / step) * ((params.ny + step - 1) / step) * 8;
+        float* vbo = nullptr;
+        if (hipMalloc(&vbo, (size_t)max_vertices * 5 * sizeof(float)) != hipSuccess) return 4;
+        const float cell = std::min(2.0f / params.nx, 2.0f / params.ny);
+        const int nv = solver.generateVectorArrows(vbo, max_vertices, step, params.u_inf, cell * (step * 0.8f), 0.5f, 0.3f);
+        std::vector<float> hv((size_t)max_vertices * 5);
+        host_copy(hv, vbo);
+        double ssum = 0;
+        for (int q = 0; q < nv * 5; q++) ssum += hv[q];
+        std::printf("arrow_vertices %d\narrow_sum %.12e\n", nv, ssum);
+        (void)hipFree(vbo);
     }
     std::printf("maxT %.9e\nmaxMach %.9e\n", solver.getMaxTemperature(), solver.getMaxMach());
     std::printf("memory %zu\n", solver.getSimulationMemoryUsage());

@@ -418,6 +418,15 @@ int fvm_max_mach(fvm_solver* s, float* out) {
     return s->eng->stat(s->case_id, 1, out);
 }
 
+int fvm_generate_vector_arrows(fvm_solver* s, float* dev_vertex_data, int max_vertices, int step, float u_inf,
+                               float max_arrow_length, float arrow_head_angle, float arrow_head_length,
+                               int* num_vertices) {
+    CHECK_SOLVER(s);
+    if (!num_vertices) return set_error(FVM_ERR_INVALID_ARGUMENT, "null output");
+    return s->eng->arrows(s->case_id, dev_vertex_data, max_vertices, step, u_inf, max_arrow_length, arrow_head_angle,
+                          arrow_head_length, num_vertices);
+}
+
 int fvm_memory_usage(fvm_solver* s, size_t* bytes) {
     if (!s || !bytes) return set_error(FVM_ERR_INVALID_ARGUMENT, "null argument");
     *bytes = s->eng ? s->eng->bytes_in_use() : 0;

@@ -2,6 +2,7 @@
 #include "fvm_engine.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1234,6 +1235,32 @@ int Engine::stat(int id, int mode, float* out) {
     FVM_HIP(hipStreamSynchronize(stream_));
     std::memcpy(&bits, h_pinned_ + 4, 4);
     std::memcpy(out, &bits, 4);
+    return FVM_OK;
+}
+
+int Engine::arrows(int id, float* dev_out, int max_vertices, int step, float u_inf, float max_len, float head_angle,
+                   float head_ratio, int* num_vertices) {
+    HostCase* hc = get(id);
+    if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
+    if (!dev_out || step < 1 || max_vertices < 0 || !(u_inf > 0.0f))
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "arrows: need a buffer, step >= 1, max_vertices >= 0 and u_inf > 0");
+    const int nax = (hc->p.nx + step - 1) / step, nay = (hc->p.ny + step - 1) / step;
+    if ((long long)nax * nay * 8 > (long long)INT32_MAX)
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "arrows: more than 2^31 vertices at this step; use a larger step");
+    // like field(): ordered after the caller's null-stream work (graphics interop)
+    FVM_HIP(hipEventRecord(ev_null_, nullptr));
+    FVM_HIP(hipStreamWaitEvent(stream_, ev_null_, 0));
+    unsigned* counter = d_stat_ + 1;
+    launch_zero_u32(stream_, counter, 1);
+    launch_arrows(fp64_, stream_, d_cases_, id, current_set(*hc), step, nax, nay, u_inf, max_len, std::cos(head_angle),
+                  std::sin(head_angle), head_ratio, max_vertices, dev_out, counter);
+    FVM_HIP(hipGetLastError());
+    FVM_HIP(hipMemcpyAsync(h_pinned_ + 8, counter, 4, hipMemcpyDeviceToHost, stream_));
+    FVM_HIP(hipStreamSynchronize(stream_));
+    unsigned n = 0;
+    std::memcpy(&n, h_pinned_ + 8, 4);
+    const unsigned cap = (unsigned)(max_vertices / 8) * 8u;
+    *num_vertices = (int)std::min(n, cap);
     return FVM_OK;
 }
 

@@ -119,6 +119,10 @@ public:
     int queue_maxima(int id);
     int field(int id, int field, float* dev_out);
     int stat(int id, int mode, float* out);
+    // generateVectorArrows (solver.cu:3809-3845) into caller device memory;
+    // ordered after the caller's null-stream work, complete on return
+    int arrows(int id, float* dev_out, int max_vertices, int step, float u_inf, float max_len, float head_angle,
+               float head_ratio, int* num_vertices);
     int conservation(int id, double out[4]);
     int forces(int id, double out[3]);   // Fx, Fy, number of wall faces
     int save_case(int id, const char* path);   // snapshot (fvm_snapshot_header layout)

@@ -1082,6 +1082,66 @@ __global__ __launch_bounds__(NT) void k_field(const CaseDesc<T>* __restrict__ ca
     }
 }
 
+// Velocity arrows for the display overlay (generateVectorArrowsKernel,
+// solver.cu:3689-3796): one candidate per sample cell (i, j) = (a step + step/2,
+// b step + step/2); a cell moving at less than 1e-6 u_inf gets none.  An arrow
+// is 8 vertices (x, y, r, g, b) in normalised device coordinates, drawn as 4
+// lines: the shaft, the two head strokes (black) and a zero-length line.  Slots come from an atomic counter, one
+// atomic per wave; vertices beyond max_vertices are not written (the counter
+// still counts them), so the caller's buffer is never overrun.  The velocity
+// is that of the primitives, u = rho_u / max(rho, rho_min).
+template <class T>
+__global__ __launch_bounds__(NT) void k_arrows(const CaseDesc<T>* __restrict__ cases, int case_id, int set, int step,
+                                               int nax, int nay, float u_inf, float max_len, float cos_a, float sin_a,
+                                               float head_ratio, int max_vertices, float* __restrict__ out,
+                                               unsigned* counter) {
+    const CaseDesc<T>& cd = cases[case_id];
+    const CaseConsts<T>& k = cd.c;
+    const int t = blockIdx.x * NT + threadIdx.x;
+    const int a = t % nax, b = t / nax;
+    const int i = a * step + step / 2, j = b * step + step / 2;
+    bool have = t < nax * nay && i < k.nx && j < k.ny;
+    float uu = 0.0f, vv = 0.0f, speed = 0.0f;
+    if (have) {
+        const size_t g = (size_t)j * k.nx + i;
+        const T r = tmax(((const T*)cd.b.U[set][0])[g], (T)k.min_rho);
+        uu = (float)(((const T*)cd.b.U[set][1])[g] / r);
+        vv = (float)(((const T*)cd.b.U[set][2])[g] / r);
+        speed = sqrtf(uu * uu + vv * vv);
+        have = !(speed < 1e-6f * u_inf);
+    }
+    // wave-aggregated slot reservation: one atomic per wave, lanes in lane order
+    const unsigned long long mask = __ballot(have);
+    const int lane = threadIdx.x & 63;
+    const unsigned before = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+    unsigned base = 0;
+    if (lane == 0 && mask) base = atomicAdd(counter, 8u * (unsigned)__popcll(mask));
+    base = (unsigned)__shfl((int)base, 0, 64);
+    if (!have) return;
+    const unsigned slot = base + 8u * before;
+    if ((long long)slot + 8 > (long long)max_vertices) return;
+    const float norm = fminf(speed / (u_inf * 1.5f), 1.0f);
+    const float sx = (float)i / k.nx * 2.0f - 1.0f, sy = (float)j / k.ny * 2.0f - 1.0f;
+    const float dx = uu / speed, dy = vv / speed;
+    const float len = max_len * norm;
+    const float ex = sx + dx * len, ey = sy + dy * len;
+    const float hl = len * head_ratio;
+    const float h1x = ex - hl * (dx * cos_a - dy * sin_a), h1y = ey - hl * (dx * sin_a + dy * cos_a);
+    const float h2x = ex - hl * (dx * cos_a + dy * sin_a), h2y = ey - hl * (-dx * sin_a + dy * cos_a);
+    // The reference reserves 8 vertices per arrow and writes the first 6, so the
+    // fourth line it draws comes from whatever the buffer held; here it is a
+    // zero-length line at the tip, which draws nothing.
+    const float xy[8][2] = {{sx, sy}, {ex, ey}, {ex, ey}, {h1x, h1y}, {ex, ey}, {h2x, h2y}, {ex, ey}, {ex, ey}};
+    float* o = out + (size_t)slot * 5;
+    for (int q = 0; q < 8; q++) {
+        o[q * 5 + 0] = xy[q][0];
+        o[q * 5 + 1] = xy[q][1];
+        o[q * 5 + 2] = 0.0f;
+        o[q * 5 + 3] = 0.0f;
+        o[q * 5 + 4] = 0.0f;
+    }
+}
+
 // Statistics: max T of the primitives (mode 0) or max Mach |v|/(c+1e-10) (mode 1)
 // (launchComputeMaxTemperature / launchComputeMaxMach, solver.cu:2782-2867).
 template <class T>
@@ -1324,6 +1384,20 @@ void launch_field(bool fp64, hipStream_t s, const void* c, int case_id, size_t n
     dim3 g((unsigned)std::min<size_t>((ncells + NT - 1) / NT, 65535));
     if (fp64) k_field<double><<<g, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set, field, out);
     else k_field<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, field, out);
+}
+
+void launch_arrows(bool fp64, hipStream_t s, const void* c, int case_id, int set, int step, int nax, int nay,
+                   float u_inf, float max_len, float cos_a, float sin_a, float head_ratio, int max_vertices, float* out,
+                   unsigned* counter) {
+    const long long n = (long long)nax * nay;
+    if (n <= 0) return;
+    dim3 g((unsigned)((n + NT - 1) / NT));
+    if (fp64)
+        k_arrows<double><<<g, NT, 0, s>>>((const CaseDesc<double>*)c, case_id, set, step, nax, nay, u_inf, max_len, cos_a,
+                                          sin_a, head_ratio, max_vertices, out, counter);
+    else
+        k_arrows<float><<<g, NT, 0, s>>>((const CaseDesc<float>*)c, case_id, set, step, nax, nay, u_inf, max_len, cos_a,
+                                         sin_a, head_ratio, max_vertices, out, counter);
 }
 
 void launch_stat(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int mode, unsigned* target) {

@@ -39,6 +39,12 @@ void launch_init(bool fp64, hipStream_t s, const void* c, int case_id, size_t nc
 void launch_copy_set(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int src, int dst);
 void launch_zero_u32(hipStream_t s, unsigned* p, int n);
 void launch_field(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int field, float* out);
+// Velocity arrows of set `set` over the nax x nay sample cells (every step-th
+// cell) into out, at most max_vertices vertices; *counter (zeroed by the
+// caller) ends as the vertices the arrows need.
+void launch_arrows(bool fp64, hipStream_t s, const void* c, int case_id, int set, int step, int nax, int nay,
+                   float u_inf, float max_len, float cos_a, float sin_a, float head_ratio, int max_vertices, float* out,
+                   unsigned* counter);
 void launch_stat(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set, int mode, unsigned* target);
 // FP64 totals of four planes of n cells (FP32 or FP64 values): nblocks
 // partials of 4 doubles in `partial`, then their fixed-order sum into out[4].

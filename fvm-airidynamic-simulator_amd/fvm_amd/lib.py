@@ -67,6 +67,8 @@ SIGNATURES = [
     ("fvm_load_snapshot", _i, [_vp, ctypes.c_char_p, _P]),
     ("fvm_max_temperature", _i, [_vp, _fp]),
     ("fvm_max_mach", _i, [_vp, _fp]),
+    ("fvm_generate_vector_arrows", _i, [_vp, _vp, _i, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.POINTER(_i)]),
     ("fvm_memory_usage", _i, [_vp, _szp]),
     ("fvm_gpu_memory_info", _i, [_i, _szp, _szp]),
     ("fvm_synchronize", _i, [_vp]),

@@ -112,6 +112,17 @@ class CFDSolver:
         check(lib().fvm_max_mach(self._h, ctypes.byref(v)), "fvm_max_mach")
         return v.value
 
+    def generateVectorArrows(self, dev_vertex_data: int, max_vertices: int, step: int, u_inf: float,
+                             max_arrow_length: float, arrow_head_angle: float, arrow_head_length: float) -> int:
+        """generateVectorArrows (solver.cu:3809-3845): GL_LINES vertices
+        (x, y, r, g, b) of the velocity arrows into caller device memory;
+        returns the vertex count."""
+        n = ctypes.c_int()
+        check(lib().fvm_generate_vector_arrows(self._h, ctypes.c_void_p(dev_vertex_data), int(max_vertices), int(step),
+                                               u_inf, max_arrow_length, arrow_head_angle, arrow_head_length,
+                                               ctypes.byref(n)), "fvm_generate_vector_arrows")
+        return n.value
+
     def getSimulationMemoryUsage(self) -> int:
         v = ctypes.c_size_t()
         check(lib().fvm_memory_usage(self._h, ctypes.byref(v)), "fvm_memory_usage")

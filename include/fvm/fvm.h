@@ -257,6 +257,18 @@ int fvm_save_snapshot(fvm_solver* s, const char* path);
  * fvm_initialize), then restores state, time, step count, CFL maxima and
  * carries.  params_out (may be NULL) receives the stored parameters. */
 int fvm_load_snapshot(fvm_solver* s, const char* path, fvm_params* params_out);
+/* generateVectorArrows (solver.cu:3809-3845): velocity arrows for the display
+ * overlay into caller device memory (an OpenGL VBO mapped by the caller), as
+ * GL_LINES vertices (x, y, r, g, b) in normalised device coordinates, 8 per
+ * arrow, one arrow per step-th cell (from step/2) that moves at 1e-6 u_inf or
+ * more; *num_vertices receives the vertices written.  Unlike the reference,
+ * nothing is written beyond max_vertices (the count is capped), and the two
+ * vertices per arrow the reference leaves unwritten form a zero-length line.
+ * Arrow order is unspecified.  Ordered after the caller's null-stream work and
+ * complete on return, as fvm_field_to_device. */
+int fvm_generate_vector_arrows(fvm_solver* s, float* dev_vertex_data, int max_vertices, int step, float u_inf,
+                               float max_arrow_length, float arrow_head_angle, float arrow_head_length,
+                               int* num_vertices);
 /* getMaxTemperature / getMaxMach (solver.cu:3417-3426). */
 int fvm_max_temperature(fvm_solver* s, float* out);
 int fvm_max_mach(fvm_solver* s, float* out);

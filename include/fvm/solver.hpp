@@ -160,7 +160,13 @@ public:
     // Velocity-arrow vertices for the OpenGL overlay (src/solver.cu:3809-3845)
     // belong to the display path, which this library does not carry; returns -1
     // ("failed"), the value the reference documents for failure.
-    int generateVectorArrows(float*, int, int, float, float, float, float) { return -1; }
+    int generateVectorArrows(float* dev_vertex_data, int max_vertices, int step, float u_inf, float max_arrow_length,
+                             float arrow_head_angle, float arrow_head_length) {
+        int n = 0;
+        FVM_CHECK(fvm_generate_vector_arrows(h_, dev_vertex_data, max_vertices, step, u_inf, max_arrow_length,
+                                             arrow_head_angle, arrow_head_length, &n));
+        return n;
+    }
     int getNx() const { return nx_; }
     int getNy() const { return ny_; }
     float computeStableTimeStep(const SimParams& params) {

@@ -5,7 +5,7 @@ apps/front_end_sequence.cpp (built beside the library) calls CFDSolver in the
 order of the reference front end (main.cpp:203-368): initialize, getCellTypes,
 reset, convertCellTypesToDevice, updateObstacleGeometry, computeStableTimeStep,
 step, the *ToDevice readbacks into caller-owned device memory, queue/read dt,
-the maxima and the memory figure.  The same sequence through the Python
+the velocity arrows, the maxima and the memory figure.  The same sequence through the Python
 binding must give the same numbers, and the readbacks must agree with each
 other.
 """
@@ -58,7 +58,17 @@ def python_sequence():
             s.step(p)
         s.synchronize()
         stats = (s.getMaxTemperature(), s.getMaxMach())
-        return dt, {f: s.field(f).astype(np.float64) for f in range(5)}, s.state(), stats
+        fields = {f: s.field(f).astype(np.float64) for f in range(5)}
+        # the front end's arrow call (main.cpp:328-346): density 20, length 0.8
+        # cell spacings per step, head 0.5 rad and 0.3 of the shaft
+        from hip_mem import DeviceBuffer
+        step = 20
+        cap = ((p.nx + step - 1) // step) * ((p.ny + step - 1) // step) * 8
+        with DeviceBuffer(cap * 5 * 4) as buf:
+            nv = s.generateVectorArrows(buf.address, cap, step, p.u_inf, min(2.0 / p.nx, 2.0 / p.ny) * (step * 0.8),
+                                        0.5, 0.3)
+            arrows = buf.download(np.empty(nv * 5, dtype=np.float32)).astype(np.float64)
+        return dt, fields, s.state(), stats, (nv, arrows.sum())
 
 
 def test_cpp_front_end_sequence():
@@ -75,12 +85,14 @@ def test_cpp_front_end_sequence():
     assert out["memory"] > 0
     # the obstacle move zeroes the cells it uncovers (updateSDFWithFixupKernel),
     # so the first CFL dt is tiny; both front ends must see the same one
-    dt, fields, state, (max_t, max_mach) = python_sequence()
+    dt, fields, state, (max_t, max_mach), (n_arrow, arrow_sum) = python_sequence()
     assert out["dt"] == pytest.approx(dt, rel=1e-6)
     # getMaxTemperature / getMaxMach reduce the clamped primitives (T from
     # E - kinetic energy, solver.cu:3417-3426); the T readback divides rho_e
     # (computeTemperatureDirectKernel), which the viscous half steps move
     # apart from E - kinetic energy near walls, so the two maxima differ.
+    assert out["arrow_vertices"] == n_arrow > 0
+    assert out["arrow_sum"] == pytest.approx(arrow_sum, rel=1e-6, abs=1e-6)
     assert out["maxT"] == pytest.approx(max_t, rel=1e-6)
     assert out["maxMach"] == pytest.approx(max_mach, rel=1e-6)
     assert out["maxMach"] == pytest.approx(out["max_mach"], rel=1e-4)

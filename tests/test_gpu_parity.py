@@ -532,3 +532,69 @@ def test_reference_si_configurations(solver, kind):
     o.step(100, p.copy())
     field_parity(solver.state64(), o.state(), oracle_state(p, 100))
     o.close()
+
+
+def expected_arrows(p, s, step, u_inf, max_len, angle, ratio):
+    """generateVectorArrowsKernel (solver.cu:3689-3796) in numpy float32 on the
+    state's primitive velocity u = rho_u / max(rho, rho_min): (n, 8, 5)."""
+    f = np.float32
+    out = []
+    for j in range(step // 2, p.ny, step):
+        for i in range(step // 2, p.nx, step):
+            r = max(f(s[0, j, i]), f(p.min_density))
+            u, v = f(s[1, j, i]) / r, f(s[2, j, i]) / r
+            speed = np.sqrt(u * u + v * v, dtype=f)
+            if speed < f(1e-6) * f(u_inf):
+                continue
+            norm = min(speed / (f(u_inf) * f(1.5)), f(1.0))
+            sx, sy = f(i) / f(p.nx) * f(2) - f(1), f(j) / f(p.ny) * f(2) - f(1)
+            dx, dy = u / speed, v / speed
+            ln = f(max_len) * norm
+            ex, ey = sx + dx * ln, sy + dy * ln
+            hl = ln * f(ratio)
+            ca, sa = f(np.cos(f(angle))), f(np.sin(f(angle)))
+            h1 = (ex - hl * (dx * ca - dy * sa), ey - hl * (dx * sa + dy * ca))
+            h2 = (ex - hl * (dx * ca + dy * sa), ey - hl * (-dx * sa + dy * ca))
+            pts = [(sx, sy), (ex, ey), (ex, ey), h1, (ex, ey), h2, (ex, ey), (ex, ey)]
+            out.append([[x, y, 0.0, 0.0, 0.0] for x, y in pts])
+    return np.array(out, dtype=np.float64).reshape(-1, 8, 5)
+
+
+def sort_arrows(a):
+    return a[np.lexsort((a[:, 0, 1], a[:, 0, 0]))]
+
+
+def test_vector_arrows_match_the_reference_formulas(solver):
+    """generateVectorArrows (solver.cu:3809-3845) into device memory the test
+    owns, with the front end's arrow constants (main.cpp:124-126, 328-346):
+    the set of arrows equals the reference formulas on the same state (cells
+    at rest inside the body get none); with a buffer for 5 arrows plus 3
+    vertices, exactly 5 arrows are written and nothing beyond them."""
+    from hip_mem import DeviceBuffer
+    p = P.euler_bow_shock()
+    p.dt = 0.1
+    announce("velocity arrows, Mach-3 bow shock 256x128 after 30 steps")
+    solver.initialize(p)
+    solver.advance(p.copy(), 30)
+    s = solver.state()
+    step = 8
+    max_len = min(2.0 / p.nx, 2.0 / p.ny) * (step * 0.8)
+    want = sort_arrows(expected_arrows(p, s, step, p.u_inf, max_len, 0.5, 0.3))
+    samples = ((p.nx + step - 1) // step) * ((p.ny + step - 1) // step)
+    assert 0 < len(want) < samples                  # the solid cells at rest are skipped
+    with DeviceBuffer(samples * 8 * 5 * 4 + 4096) as buf:
+        nv = solver.generateVectorArrows(buf.address, samples * 8, step, p.u_inf, max_len, 0.5, 0.3)
+        assert nv == 8 * len(want)
+        got = buf.download(np.empty((nv // 8, 8, 5), dtype=np.float32)).astype(np.float64)
+        got = sort_arrows(got)
+        assert np.abs(got - want).max() < 1e-6, np.abs(got - want).max()
+        # capacity: room for 5 arrows and 3 vertices; the rest of the buffer keeps its sentinel
+        sentinel = np.full(samples * 8 * 5, 7.5, dtype=np.float32)
+        buf.upload(sentinel)
+        nv = solver.generateVectorArrows(buf.address, 43, step, p.u_inf, max_len, 0.5, 0.3)
+        assert nv == 40
+        back = buf.download(np.empty(samples * 8 * 5, dtype=np.float32))
+        assert np.all(back[nv * 5:] == 7.5)
+        five = back[:nv * 5].astype(np.float64).reshape(5, 8, 5)
+        for a in five:
+            assert np.abs(want - a).max(axis=(1, 2)).min() < 1e-6   # each written arrow is one of the set

@@ -10,25 +10,11 @@ import pytest
 
 import fvm_amd
 from fvm_amd import params as P
+from hip_mem import D2H, H2D, hip_runtime
 
 pytestmark = pytest.mark.gpu
 
 
-def hip_runtime():
-    """The HIP runtime the library itself loaded (same library, same process
-    state), for device buffers owned by the test."""
-    fvm_amd.lib()
-    with open("/proc/self/maps") as f:
-        paths = {line.split()[-1] for line in f if "libamdhip64" in line}
-    assert paths, "libamdhip64 not loaded by libfvm_amd"
-    hip = ctypes.CDLL(sorted(paths)[0])
-    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-    hip.hipFree.argtypes = [ctypes.c_void_p]
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    return hip
-
-
-H2D, D2H = 1, 2
 FVM_ERR_INVALID_ARGUMENT = -1   # fvm.h
 
 
