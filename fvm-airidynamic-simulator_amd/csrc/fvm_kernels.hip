@@ -392,31 +392,31 @@ __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, 
 // KAHAN: the final blend is written as U^n + delta with a carried compensation.
 // ri / rv: roles for inviscid / viscous cases.
 // ---------------------------------------------------------------------------
-// Waves per SIMD asked of the register allocator for k_stage: six for FP32
-// (at most 80 VGPRs; measured faster than the five an unconstrained allocation
-// allows) unless the tile's LDS already admits fewer workgroups per CU (160
-// KiB of LDS, one wave per SIMD per 256-thread workgroup); FP64 keeps its
-// registers.
-template <class T, int TX, int TY>
+// Waves per SIMD asked of the register allocator for k_stage: seven for FP32
+// (at most 72 VGPRs; the 64x7 stage kernels need 63-65) and six for the
+// compensated variant (80 VGPRs: at 72 it spills), unless the tile's LDS
+// already admits fewer workgroups per CU (160 KiB of LDS, one wave per SIMD
+// per 256-thread workgroup); FP64 keeps its registers.
+template <class T, int TX, int TY, bool KAHAN>
 constexpr int stage_waves() {
     constexpr int SX = TX + 4, SY = TY + 4, FX = (TX + 1) * TY, FY = TX * (TY + 1);
     constexpr int FMAX = FX > FY ? FX : FY;
-    constexpr int lds = 4 * SX * SY * (int)sizeof(T) + SX * SY + 6 * FMAX * (int)sizeof(T) + 64;
+    constexpr int lds = 4 * SX * SY * (int)sizeof(T) + SX * SY + 5 * FMAX * (int)sizeof(T) + 64;
     constexpr int by_lds = 163840 / lds;
-    return sizeof(T) != 4 ? 1 : (by_lds < 6 ? by_lds : 6);
+    constexpr int cap = KAHAN ? 6 : 7;
+    return sizeof(T) != 4 ? 1 : (by_lds < cap ? by_lds : cap);
 }
 
 template <class T, int TX, int TY, int MODE, bool KAHAN>
-__global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
+__global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, KAHAN>())) void k_stage(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
                                               const int* __restrict__ count, Roles ri, Roles rv) {
     constexpr int SX = TX + 4, SY = TY + 4, CPT = (TX * TY + NT - 1) / NT;
     constexpr int FX = (TX + 1) * TY, FY = TX * (TY + 1);
     constexpr int FMAX = FX > FY ? FX : FY;
     __shared__ T s_r[SY * SX], s_u[SY * SX], s_v[SY * SX], s_p[SY * SX];
     __shared__ uint8_t s_ct[SY * SX];
-    // five flux components + the interface velocity of the -p div(v) source
-    // (mass flux over the mean clamped density of the two cells, solver.cu:1787-1794)
-    __shared__ T s_f[6][FMAX];
+    // five flux components per face
+    __shared__ T s_f[5][FMAX];
     // CFL maxima: FP64 for the FP64 state and the compensated variant
     constexpr bool DT64 = KAHAN || sizeof(T) == 8;
     using MT = typename std::conditional<DT64, double, float>::type;
@@ -435,6 +435,10 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
         const double dtd = dt_from_slot<DT64>(cd, r.slot_r);
         const T dt = (T)dtd;
 
+        // thread index opaque per tile (indices derived from it are not held
+        // in registers across the tile loop)
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
         if (it.index == 0 && threadIdx.x == 0) {
             if (r.zero_slot) {
                 cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
@@ -448,7 +452,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
         const Plane<uint8_t> ctp(cd.b.ct);
 
         // ---- stage tile + halo 2: conserved -> (rho, u, v, p) in LDS
-        for (int n = threadIdx.x; n < SX * SY; n += NT) {
+        for (int n = tid; n < SX * SY; n += NT) {
             int li = n % SX, lj = n / SX;
             int gi = min(max(i0 + li - 2, 0), nx - 1);
             int gj = min(max(j0 + lj - 2, 0), ny - 1);
@@ -482,11 +486,10 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
         __syncthreads();
 
         // ---- x faces i+1/2 for local i in [-1, TX-1]
-        for (int n = threadIdx.x; n < FX; n += NT) {
+        for (int n = tid; n < FX; n += NT) {
             int fi = n % (TX + 1) - 1, fj = n / (TX + 1);
             int gi = i0 + fi, gj = j0 + fj;
             T o[5] = {T(0), T(0), T(0), T(0), T(0)};
-            T uf = T(0);
             if (gi >= 0 && gi < nx - 1 && gj < ny) {
                 int b = (fj + 2) * SX + (fi + 1);
                 T rr[4], un[4], ut[4], pp[4];
@@ -496,35 +499,33 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
                     rr[m] = s_r[b + m]; un[m] = s_u[b + m]; ut[m] = s_v[b + m]; pp[m] = s_p[b + m]; cc[m] = s_ct[b + m];
                 }
                 face_flux(k, rr, un, ut, pp, cc, o);
-                uf = T(2) * o[0] * Num<T>::rcp(tmax(rr[1], k.min_rho) + tmax(rr[2], k.min_rho));
             }
 #pragma unroll
             for (int c = 0; c < 5; c++) s_f[c][n] = o[c];
-            s_f[5][n] = uf;
         }
         __syncthreads();
 
         T res[CPT][5];
-        T fxr[CPT], fxl[CPT], fyt[CPT], fyb[CPT];
+        // x-face mass fluxes of the -p div(v) source term
+        T mxr[CPT], mxl[CPT];
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
-            int n = threadIdx.x + q * NT;
+            int n = tid + q * NT;
             if (n >= TX * TY) break;
             int li = n % TX, lj = n / TX;
             int a = lj * (TX + 1) + li;
 #pragma unroll
             for (int c = 0; c < 5; c++) res[q][c] = (s_f[c][a + 1] - s_f[c][a]) * (T)k.inv_dx;
-            fxr[q] = s_f[5][a + 1];
-            fxl[q] = s_f[5][a];
+            mxr[q] = s_f[0][a + 1];
+            mxl[q] = s_f[0][a];
         }
         __syncthreads();
 
         // ---- y faces j+1/2 for local j in [-1, TY-1]
-        for (int n = threadIdx.x; n < FY; n += NT) {
+        for (int n = tid; n < FY; n += NT) {
             int fi = n % TX, fj = n / TX - 1;
             int gi = i0 + fi, gj = j0 + fj;
             T o[5] = {T(0), T(0), T(0), T(0), T(0)};
-            T uf = T(0);
             if (gj >= 0 && gj < ny - 1 && gi < nx) {
                 int b = (fj + 1) * SX + (fi + 2);
                 T rr[4], un[4], ut[4], pp[4];
@@ -535,17 +536,15 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
                     rr[m] = s_r[s]; un[m] = s_v[s]; ut[m] = s_u[s]; pp[m] = s_p[s]; cc[m] = s_ct[s];
                 }
                 face_flux(k, rr, un, ut, pp, cc, o);
-                uf = T(2) * o[0] * Num<T>::rcp(tmax(rr[1], k.min_rho) + tmax(rr[2], k.min_rho));
             }
             s_f[0][n] = o[0]; s_f[1][n] = o[2]; s_f[2][n] = o[1]; s_f[3][n] = o[3]; s_f[4][n] = o[4];
-            s_f[5][n] = uf;
         }
         __syncthreads();
 
         MT vmax_local = MT(0);
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
-            int n = threadIdx.x + q * NT;
+            int n = tid + q * NT;
             if (n >= TX * TY) break;
             int li = n % TX, lj = n / TX;
             int gi = i0 + li, gj = j0 + lj;
@@ -553,8 +552,6 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
             T dfy[5];
 #pragma unroll
             for (int c = 0; c < 5; c++) dfy[c] = (s_f[c][a + TX] - s_f[c][a]) * (T)k.inv_dy;
-            fyt[q] = s_f[5][a + TX];
-            fyb[q] = s_f[5][a];
             if (gi >= nx || gj >= ny) continue;
             const unsigned g = (unsigned)(gj * nx + gi);
             const int sc = (lj + 2) * SX + (li + 2);
@@ -569,7 +566,14 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY>())) void k_stage(const 
                 const bool ex = (gi == 0 || gi == nx - 1), ey = (gj == 0 || gj == ny - 1);
                 T src = T(0);
                 if (!ex && !ey) {
-                    T divv = (fxr[q] - fxl[q]) * (T)k.inv_dx + (fyt[q] - fyb[q]) * (T)k.inv_dy;
+                    // interface velocities: face mass flux over the mean clamped
+                    // density of the face's two cells (solver.cu:1787-1794)
+                    const T rc = tmax(s_r[sc], (T)k.min_rho);
+                    const T fxr = T(2) * mxr[q] * Num<T>::rcp(rc + tmax(s_r[sc + 1], (T)k.min_rho));
+                    const T fxl = T(2) * mxl[q] * Num<T>::rcp(tmax(s_r[sc - 1], (T)k.min_rho) + rc);
+                    const T fyt = T(2) * s_f[0][a + TX] * Num<T>::rcp(rc + tmax(s_r[sc + SX], (T)k.min_rho));
+                    const T fyb = T(2) * s_f[0][a] * Num<T>::rcp(tmax(s_r[sc - SX], (T)k.min_rho) + rc);
+                    T divv = (fxr - fxl) * (T)k.inv_dx + (fyt - fyb) * (T)k.inv_dy;
                     src = -s_p[sc] * divv;
                 }
                 T d[5];
@@ -886,6 +890,11 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
         const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
         const double dtd = dt_from_slot<(sizeof(T) == 8)>(cd, r.slot_r);
         const T hdt = (T)(dtd * 0.5);
+        // The thread index, opaque to the compiler per tile: every index
+        // derived from it is recomputed per tile rather than held in registers
+        // across the tile loop (which spilled them).
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
         if (it.index == 0 && threadIdx.x == 0) {
             if (r.zero_slot) {
                 cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
@@ -899,19 +908,39 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
                                   Plane<T>(cd.b.U[r.out][3]), Plane<T>(cd.b.U[r.out][4])};
         const Plane<uint8_t> ctp(cd.b.ct);
 
-        for (int n = threadIdx.x; n < SX * SY; n += NT) {
-            int li = n % SX, lj = n / SX;
-            int gi = min(max(i0 + li - 2, 0), nx - 1);
-            int gj = min(max(j0 + lj - 2, 0), ny - 1);
-            const unsigned g = (unsigned)(gj * nx + gi);
-            T u, v, p, tt;
-            primitives(k, in[0][g], in[1][g], in[2][g], in[3][g], in[4][g], u, v, p, tt);
-            s_u[n] = u; s_v[n] = v; s_T[n] = tt; s_ct[n] = ctp[g];
+        // Every load of the tile is issued before the first use: the staged
+        // cells' states and tags, and the states the update below starts
+        // from, so that the tile waits for one memory latency, not one per
+        // staging pass.
+        constexpr int SN = (SX * SY + NT - 1) / NT;
+        T sq[SN][5];
+        uint8_t sc[SN];
+#pragma unroll
+        for (int m = 0; m < SN; m++) {
+            const int n = tid + m * NT;
+            if (n < SX * SY) {
+                int li = n % SX, lj = n / SX;
+                int gi = min(max(i0 + li - 2, 0), nx - 1);
+                int gj = min(max(j0 + lj - 2, 0), ny - 1);
+                const unsigned g = (unsigned)(gj * nx + gi);
+#pragma unroll
+                for (int f = 0; f < 5; f++) sq[m][f] = in[f][g];
+                sc[m] = ctp[g];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < SN; m++) {
+            const int n = tid + m * NT;
+            if (n < SX * SY) {
+                T u, v, p, tt;
+                primitives(k, sq[m][0], sq[m][1], sq[m][2], sq[m][3], sq[m][4], u, v, p, tt);
+                s_u[n] = u; s_v[n] = v; s_T[n] = tt; s_ct[n] = sc[m];
+            }
         }
         __syncthreads();
 
         const T i2x = (T)k.inv_2dx, i2y = (T)k.inv_2dy, ix = (T)k.inv_dx, iy = (T)k.inv_dy;
-        for (int n = threadIdx.x; n < QX * QY; n += NT) {
+        for (int n = tid; n < QX * QY; n += NT) {
             int ti = n % QX, tj = n / QX;
             int gi = i0 + ti - 1, gj = j0 + tj - 1;
             int s = (tj + 1) * SX + (ti + 1);
@@ -948,7 +977,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
         T smax = T(0), numax = T(0);
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
-            int n = threadIdx.x + q * NT;
+            int n = tid + q * NT;
             if (n >= TX * TY) break;
             int li = n % TX, lj = n / TX;
             int gi = i0 + li, gj = j0 + lj;
