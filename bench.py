@@ -35,6 +35,7 @@ VALU_PEAK = 256 * 4 * 16 * 2.4e9
 KERNEL_BYTES_PER_CELL = {
     "diffuse_half1": 20 + 1 + 16,        # read U^n, tag; write U_d (rho is U^n's plane)
     "stage1": 20 + 1 + 20,               # read U_d (or U^n), tag; write U*
+    "stage1_fused": 20 + 1 + 16 + 20,    # read U^n, tag; write U_d and U* (fused schedule)
     "stage2": 20 + 20 + 1 + 20,          # read U*, U_d (blend), tag; write blend
     "diffuse_half2": 20 + 1 + 20,        # read blend, tag; write U^{n+1}
 }

@@ -28,6 +28,7 @@ struct fvm_solver {
     fvm_params p{};
     bool have_params = false;
     int tile_x = 0, tile_y = 0;
+    int schedule = FVM_SCHEDULE_AUTO;
     bool timing = false;
 };
 
@@ -37,6 +38,8 @@ struct fvm_batch {
     size_t budget = 0;
     int max_cases = 0;
     bool use_graph = true;
+    int schedule = FVM_SCHEDULE_AUTO;
+    bool default_tile = true;
 };
 
 namespace {
@@ -54,13 +57,14 @@ bool consts_differ(const fvm_params& a, const fvm_params& b) {
 #include "tile_table.inc"
 
 struct TileEntry {
-    int viscous, bucket, tx, ty;
+    int viscous, bucket, tx, ty, fused;
 };
 const TileEntry kTileTable[] = {FVM_TILE_ENTRIES};
 
-// Tile shape for a case from the autotuned table: same viscosity, nearest
-// log4 cell-count bucket (fvm_amd/autotune.py measures and writes the table).
-int pick_tile(const fvm_params& p, int* tx, int* ty) {
+// Tile shape and viscous schedule for a case from the autotuned table: same
+// viscosity, nearest log4 cell-count bucket (fvm_amd/autotune.py measures and
+// writes the table).
+int pick_tile(const fvm_params& p, int* tx, int* ty, bool* fused = nullptr) {
     double cells = (double)p.nx * (double)p.ny;
     int bucket = (int)std::lround(std::log(std::max(cells, 1.0) / 32768.0) / std::log(4.0));
     bucket = std::max(0, std::min(4, bucket));
@@ -72,8 +76,27 @@ int pick_tile(const fvm_params& p, int* tx, int* ty) {
     }
     *tx = best >= 0 ? kTileTable[best].tx : 64;
     *ty = best >= 0 ? kTileTable[best].ty : 8;
+    if (fused) *fused = best >= 0 && kTileTable[best].fused != 0;
     if (!fvm::tile_supported(*tx, *ty, p.precision == FVM_PRECISION_FP64)) { *tx = 64; *ty = 8; }
     return 0;
+}
+
+// The schedule an engine runs: the explicit choice, or under AUTO the table's
+// (for an explicit tile: split).
+bool schedule_fused(const fvm_solver* s, const fvm_params& p) {
+    if (s->schedule != FVM_SCHEDULE_AUTO) return s->schedule == FVM_SCHEDULE_FUSED;
+    if (s->tile_x != 0) return false;
+    int tx, ty;
+    bool fused = false;
+    pick_tile(p, &tx, &ty, &fused);
+    return fused;
+}
+
+// The batched engine's schedule: explicit, or under AUTO the autotuned batch
+// default (for an explicit tile: split).
+bool batch_fused(const fvm_batch* b) {
+    if (b->schedule != FVM_SCHEDULE_AUTO) return b->schedule == FVM_SCHEDULE_FUSED;
+    return b->default_tile && FVM_BATCH_FUSED;
 }
 
 int ensure_engine(fvm_solver* s, const fvm_params& p) {
@@ -89,6 +112,7 @@ int ensure_engine(fvm_solver* s, const fvm_params& p) {
     if (tx == 0) pick_tile(p, &tx, &ty);
     rc = s->eng->set_tile(tx, ty);
     if (rc) return rc;
+    s->eng->set_fused_d1(schedule_fused(s, p));
     s->eng->enable_timing(s->timing);
     int id = -1;
     rc = s->eng->add_case(p, 0, true, &id);
@@ -461,6 +485,7 @@ int fvm_set_tile(fvm_solver* s, int tile_x, int tile_y) {
         if (s->eng) {
             int tx, ty;
             pick_tile(s->p, &tx, &ty);
+            s->eng->set_fused_d1(schedule_fused(s, s->p));
             return s->eng->set_tile(tx, ty);
         }
         return FVM_OK;
@@ -468,7 +493,25 @@ int fvm_set_tile(fvm_solver* s, int tile_x, int tile_y) {
     if (!fvm::tile_supported(tile_x, tile_y, false)) return set_error(FVM_ERR_INVALID_ARGUMENT, "unsupported tile shape");
     s->tile_x = tile_x;
     s->tile_y = tile_y;
-    if (s->eng) return s->eng->set_tile(tile_x, tile_y);
+    if (s->eng) {
+        s->eng->set_fused_d1(schedule_fused(s, s->p));
+        return s->eng->set_tile(tile_x, tile_y);
+    }
+    return FVM_OK;
+}
+
+int fvm_set_viscous_schedule(fvm_solver* s, int schedule) {
+    if (!s) return set_error(FVM_ERR_INVALID_ARGUMENT, "null solver handle");
+    if (schedule != FVM_SCHEDULE_AUTO && schedule != FVM_SCHEDULE_SPLIT && schedule != FVM_SCHEDULE_FUSED)
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "unknown viscous schedule");
+    s->schedule = schedule;
+    if (s->eng) s->eng->set_fused_d1(schedule_fused(s, s->p));
+    return FVM_OK;
+}
+
+int fvm_fused_schedule_supported(int tile_x, int tile_y, int precision, int* supported) {
+    if (!supported) return set_error(FVM_ERR_INVALID_ARGUMENT, "null argument");
+    *supported = precision == FVM_PRECISION_FP32 && fvm::fused_d1_supported(tile_x, tile_y, false) ? 1 : 0;
     return FVM_OK;
 }
 
@@ -514,6 +557,7 @@ int fvm_batch_create(int device, size_t mem_budget_bytes, int max_cases, fvm_bat
     if (rc) return rc;
     rc = b->eng->set_tile(FVM_BATCH_TILE_X, FVM_BATCH_TILE_Y);
     if (rc) return rc;
+    b->eng->set_fused_d1(batch_fused(b.get()));
     rc = b->eng->reserve_cases(max_cases);
     if (rc) return rc;
     rc = b->eng->set_arena(mem_budget_bytes);
@@ -648,12 +692,23 @@ int fvm_batch_synchronize(fvm_batch* b) {
     return b->eng->sync();
 }
 
+int fvm_batch_set_viscous_schedule(fvm_batch* b, int schedule) {
+    if (!b) return set_error(FVM_ERR_INVALID_ARGUMENT, "null batch");
+    if (schedule != FVM_SCHEDULE_AUTO && schedule != FVM_SCHEDULE_SPLIT && schedule != FVM_SCHEDULE_FUSED)
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "unknown viscous schedule");
+    b->schedule = schedule;
+    b->eng->set_fused_d1(batch_fused(b));
+    return FVM_OK;
+}
+
 int fvm_batch_set_tile(fvm_batch* b, int tile_x, int tile_y) {
     if (!b) return set_error(FVM_ERR_INVALID_ARGUMENT, "null batch");
-    if (tile_x == 0 && tile_y == 0) {
+    b->default_tile = tile_x == 0 && tile_y == 0;
+    if (b->default_tile) {
         tile_x = FVM_BATCH_TILE_X;
         tile_y = FVM_BATCH_TILE_Y;
     }
+    b->eng->set_fused_d1(batch_fused(b));
     return b->eng->set_tile(tile_x, tile_y);
 }
 

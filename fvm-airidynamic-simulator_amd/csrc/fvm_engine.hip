@@ -181,6 +181,16 @@ int Engine::set_tile(int tx, int ty) {
     return FVM_OK;
 }
 
+void Engine::set_fused_d1(bool on) {
+    if (on == fuse_d1_) return;
+    fuse_d1_ = on;
+    drop_graphs();
+}
+
+bool Engine::fused_d1() const {
+    return fuse_d1_ && !fp64_ && !kahan_ && fused_d1_supported(tx_, ty_, fp64_);
+}
+
 HostCase* Engine::get(int id) {
     if (id < 0 || id >= (int)cases_.size()) return nullptr;
     return cases_[id];
@@ -905,6 +915,8 @@ int Engine::enqueue_step(int ph) {
     Roles i_s2{1, nxt, cur, -1, -1, sr, sw, 0, 0};
     Roles i_g2{-1, nxt, -1, cur, 1, sr, sw, 0, 0};
     Roles v_k0{0, 3, -1, -1, -1, sr, sw, 1, 1};
+    // fused: the half step's result goes to set 3 (blend), stage 1's to set 1
+    Roles v_f1{0, 1, 3, -1, -1, sr, sw, 1, 1};
     Roles v_s1{3, 1, -1, -1, -1, sr, sw, 0, 0};
     Roles v_s2{1, 2, 3, -1, -1, sr, sw, 0, 0};
     Roles v_k3{2, 0, -1, -1, -1, sr, sw, 0, 0};
@@ -916,7 +928,8 @@ int Engine::enqueue_step(int ph) {
     kc.kahan = kahan_;
     int rc = timed_begin();
     if (rc) return rc;
-    if (any_visc_) {
+    const bool fused = any_visc_ && fused_d1();
+    if (any_visc_ && !fused) {
         kc.op = OP_DIFFUSE;
         kc.grid = dim3(grid_visc_);
         kc.work = d_tiles_visc_;
@@ -926,14 +939,14 @@ int Engine::enqueue_step(int ph) {
         launch_tiled(fp64_, tx_, ty_, kc);
         rc = timed("diffuse_half1", rc);
     }
-    kc.op = OP_STAGE1;
+    kc.op = fused ? OP_STAGE1_FUSED : OP_STAGE1;
     kc.grid = dim3(grid_tiles_);
     kc.work = d_tiles_all_;
     kc.count = d_counts_ + 0;
     kc.ri = i_s1;
-    kc.rv = v_s1;
+    kc.rv = fused ? v_f1 : v_s1;
     launch_tiled(fp64_, tx_, ty_, kc);
-    rc = timed("stage1", rc);
+    rc = timed(fused ? "stage1_fused" : "stage1", rc);
     kc.op = OP_STAGE2;
     kc.ri = i_s2;
     kc.rv = v_s2;

@@ -82,6 +82,10 @@ public:
 
     int init_stream();
     int set_tile(int tx, int ty);
+    // Viscous step schedule: false = the first half step as a launch of its
+    // own, true = fused into stage 1 (where the tile and precision have it).
+    void set_fused_d1(bool on);
+    bool fused_d1() const;
     int tile_x() const { return tx_; }
     int tile_y() const { return ty_; }
 
@@ -191,6 +195,7 @@ private:
     int single_case_ = -1;              // the one active case of a non-persistent engine, else -1
     int grid_tiles_ = 0, grid_visc_ = 0, grid_ghosts_ = 0, grid_reduce_ = 1;
     bool any_visc_ = false;
+    bool fuse_d1_ = false;
     bool persistent_ = false;
     int persistent_grid_ = 2048;
 

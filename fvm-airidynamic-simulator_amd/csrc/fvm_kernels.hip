@@ -386,37 +386,138 @@ __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, 
 #pragma clang fp contract(on)
 
 // ---------------------------------------------------------------------------
+// Viscous half step pieces (fusedViscousDiffusionKernel, src/solver.cu:2251-2511),
+// shared by the half-step kernel and the stage-1 kernel that fuses the first
+// half step in.
+// ---------------------------------------------------------------------------
+// Stress and heat flux of one cell of the ring around the updated cells, into
+// entry n of the five flux planes: Sutherland mu, k = mu cp / Pr (2317-2352),
+// central gradients of the staged u, v, T (row stride W, cell s), one-sided at
+// the domain edges (2360-2388); zero outside the domain and in solid cells.
+template <class T, int W>
+__device__ __forceinline__ void viscous_flux(const CaseConsts<T>& k, const T* s_u, const T* s_v, const T* s_T,
+                                             const uint8_t* s_ct, int s, int gi, int gj, int n, T* s_txx, T* s_tyy,
+                                             T* s_txy, T* s_qx, T* s_qy) {
+    const int nx = k.nx, ny = k.ny;
+    if (gi < 0 || gi >= nx || gj < 0 || gj >= ny || s_ct[s] == CT_SOLID) {
+        s_txx[n] = T(0); s_tyy[n] = T(0); s_txy[n] = T(0); s_qx[n] = T(0); s_qy[n] = T(0);
+        return;
+    }
+    const T i2x = (T)k.inv_2dx, i2y = (T)k.inv_2dy, ix = (T)k.inv_dx, iy = (T)k.inv_dy;
+    T Tv = s_T[s];
+    T mu = sutherland(k, Tv);
+    T kk = mu * (T)k.cp * Num<T>::rcp((T)k.prandtl);
+    T dudx = (s_u[s + 1] - s_u[s - 1]) * i2x, dvdx = (s_v[s + 1] - s_v[s - 1]) * i2x;
+    T dudy = (s_u[s + W] - s_u[s - W]) * i2y, dvdy = (s_v[s + W] - s_v[s - W]) * i2y;
+    T dTdx = (s_T[s + 1] - s_T[s - 1]) * i2x, dTdy = (s_T[s + W] - s_T[s - W]) * i2y;
+    if (gi == 0) {
+        dudx = (s_u[s + 1] - s_u[s]) * ix; dvdx = (s_v[s + 1] - s_v[s]) * ix; dTdx = (s_T[s + 1] - s_T[s]) * ix;
+    } else if (gi == nx - 1) {
+        dudx = (s_u[s] - s_u[s - 1]) * ix; dvdx = (s_v[s] - s_v[s - 1]) * ix; dTdx = (s_T[s] - s_T[s - 1]) * ix;
+    }
+    if (gj == 0) {
+        dudy = (s_u[s + W] - s_u[s]) * iy; dvdy = (s_v[s + W] - s_v[s]) * iy; dTdy = (s_T[s + W] - s_T[s]) * iy;
+    } else if (gj == ny - 1) {
+        dudy = (s_u[s] - s_u[s - W]) * iy; dvdy = (s_v[s] - s_v[s - W]) * iy; dTdy = (s_T[s] - s_T[s - W]) * iy;
+    }
+    T divv = dudx + dvdy;
+    const T two3 = T(2) / T(3);
+    s_txx[n] = mu * (T(2) * dudx - two3 * divv);
+    s_tyy[n] = mu * (T(2) * dvdy - two3 * divv);
+    s_txy[n] = mu * (dudy + dvdx);
+    s_qx[n] = -kk * dTdx;
+    s_qy[n] = -kk * dTdy;
+}
+
+// Half-step update of one interior fluid cell (2420-2505): momentum by div tau,
+// E by -div q + div(tau v), rho e by -div q + Phi, from the ring's planes (row
+// stride Q, entry t) and the staged velocities (row stride W, cell s).
+template <class T, int W, int Q>
+__device__ __forceinline__ void viscous_update(const CaseConsts<T>& k, const T* s_u, const T* s_v, int s,
+                                               const T* s_txx, const T* s_tyy, const T* s_txy, const T* s_qx,
+                                               const T* s_qy, int t, T hdt, T v[5]) {
+    const T i2x = (T)k.inv_2dx, i2y = (T)k.inv_2dy;
+    T dtxx = (s_txx[t + 1] - s_txx[t - 1]) * i2x;
+    T dtxy_y = (s_txy[t + Q] - s_txy[t - Q]) * i2y;
+    T dtxy_x = (s_txy[t + 1] - s_txy[t - 1]) * i2x;
+    T dtyy = (s_tyy[t + Q] - s_tyy[t - Q]) * i2y;
+    T dqx = (s_qx[t + 1] - s_qx[t - 1]) * i2x;
+    T dqy = (s_qy[t + Q] - s_qy[t - Q]) * i2y;
+    T dudx = (s_u[s + 1] - s_u[s - 1]) * i2x, dudy = (s_u[s + W] - s_u[s - W]) * i2y;
+    T dvdx = (s_v[s + 1] - s_v[s - 1]) * i2x, dvdy = (s_v[s + W] - s_v[s - W]) * i2y;
+    T Phi = s_txx[t] * dudx + s_tyy[t] * dvdy + s_txy[t] * (dudy + dvdx);
+    T wxp = s_u[s + 1] * s_txx[t + 1] + s_v[s + 1] * s_txy[t + 1];
+    T wxm = s_u[s - 1] * s_txx[t - 1] + s_v[s - 1] * s_txy[t - 1];
+    T wyp = s_u[s + W] * s_txy[t + Q] + s_v[s + W] * s_tyy[t + Q];
+    T wym = s_u[s - W] * s_txy[t - Q] + s_v[s - W] * s_tyy[t - Q];
+    T rhsE = -dqx - dqy + (wxp - wxm) * i2x + (wyp - wym) * i2y;
+    T rhse = -dqx - dqy + Phi;
+    v[1] += hdt * (dtxx + dtxy_y);
+    v[2] += hdt * (dtxy_x + dtyy);
+    v[3] += hdt * rhsE;
+    v[4] += hdt * rhse;
+}
+
+// ---------------------------------------------------------------------------
 // Fused RK stage.  MODE 0: stage 1 (update + point-wise BC).  MODE 1: stage 2
 // of an inviscid case (update + blend + point-wise BC + CFL maxima) or of a
 // viscous case (update + blend only, no BC: the second half step follows).
+// MODE 2: stage 1 with the first viscous half step fused in for viscous cases
+// (inviscid cases as MODE 0): the half-step result on the tile and its 2-cell
+// halo is recomputed from U^n on a 4-cell halo and staged in place of a
+// loaded state; the tile's own cells of it are stored in set r.blend, where
+// stage 2 reads its blend input.  The results equal the split launches bit
+// for bit (the same cell arithmetic, shared with k_diffuse).
 // KAHAN: the final blend is written as U^n + delta with a carried compensation.
 // ri / rv: roles for inviscid / viscous cases.
 // ---------------------------------------------------------------------------
+// LDS bytes of k_stage: staged (rho, u, v, p) + face fluxes + tags; MODE 2
+// overlays the half step's (u, v, T) + tags on the 4-cell halo and its five
+// stress / heat-flux planes on the 3-cell ring on the same space.
+template <class T, int TX, int TY, int MODE>
+constexpr int stage_lds() {
+    constexpr int SX = TX + 4, SY = TY + 4, FX = (TX + 1) * TY, FY = TX * (TY + 1);
+    constexpr int FMAX = FX > FY ? FX : FY;
+    constexpr int stage = (4 * SX * SY + 5 * FMAX) * (int)sizeof(T) + SX * SY;
+    constexpr int AXY = (TX + 8) * (TY + 8), BXY = (TX + 6) * (TY + 6);
+    constexpr int d1 = MODE == 2 ? (3 * AXY + 5 * BXY) * (int)sizeof(T) + AXY : 0;
+    return ((stage > d1 ? stage : d1) + 15) / 16 * 16;
+}
+
+// The fused first half step is built for FP32 tiles whose LDS fits a workgroup.
+template <class T, int TX, int TY>
+constexpr bool fused_d1_fits() {
+    return sizeof(T) == 4 && stage_lds<T, TX, TY, 2>() + 64 <= 65536;
+}
+
 // Waves per SIMD asked of the register allocator for k_stage: seven for FP32
 // (at most 72 VGPRs; the 64x7 stage kernels need 63-65) and six for the
 // compensated variant (80 VGPRs: at 72 it spills), unless the tile's LDS
 // already admits fewer workgroups per CU (160 KiB of LDS, one wave per SIMD
 // per 256-thread workgroup); FP64 keeps its registers.
-template <class T, int TX, int TY, bool KAHAN>
+template <class T, int TX, int TY, int MODE, bool KAHAN>
 constexpr int stage_waves() {
-    constexpr int SX = TX + 4, SY = TY + 4, FX = (TX + 1) * TY, FY = TX * (TY + 1);
-    constexpr int FMAX = FX > FY ? FX : FY;
-    constexpr int lds = 4 * SX * SY * (int)sizeof(T) + SX * SY + 5 * FMAX * (int)sizeof(T) + 64;
+    constexpr int lds = stage_lds<T, TX, TY, MODE>() + 64;
     constexpr int by_lds = 163840 / lds;
-    constexpr int cap = KAHAN ? 6 : 7;
+    constexpr int cap = (KAHAN || MODE == 2) ? 6 : 7;
     return sizeof(T) != 4 ? 1 : (by_lds < cap ? by_lds : cap);
 }
 
 template <class T, int TX, int TY, int MODE, bool KAHAN>
-__global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, KAHAN>())) void k_stage(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
+__global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k_stage(const CaseDesc<T>* __restrict__ cases, const WorkItem* __restrict__ work,
                                               const int* __restrict__ count, Roles ri, Roles rv) {
     constexpr int SX = TX + 4, SY = TY + 4, CPT = (TX * TY + NT - 1) / NT;
     constexpr int FX = (TX + 1) * TY, FY = TX * (TY + 1);
     constexpr int FMAX = FX > FY ? FX : FY;
-    __shared__ T s_r[SY * SX], s_u[SY * SX], s_v[SY * SX], s_p[SY * SX];
-    __shared__ uint8_t s_ct[SY * SX];
+    static_assert(MODE != 2 || fused_d1_fits<T, TX, TY>(), "fused half step: FP32 tiles within 64 KiB of LDS");
+    __shared__ __align__(16) char smem[stage_lds<T, TX, TY, MODE>()];
+    T* const s_r = (T*)smem;
+    T* const s_u = s_r + SX * SY;
+    T* const s_v = s_u + SX * SY;
+    T* const s_p = s_v + SX * SY;
     // five flux components per face
-    __shared__ T s_f[5][FMAX];
+    T (*const s_f)[FMAX] = (T (*)[FMAX])(s_p + SX * SY);
+    uint8_t* const s_ct = (uint8_t*)(s_p + SX * SY + 5 * FMAX);
     // CFL maxima: FP64 for the FP64 state and the compensated variant
     constexpr bool DT64 = KAHAN || sizeof(T) == 8;
     using MT = typename std::conditional<DT64, double, float>::type;
@@ -451,6 +552,69 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, KAHAN>())) void k_stage
             in4(cd.b.U[r.in][4]);
         const Plane<uint8_t> ctp(cd.b.ct);
 
+        if (MODE == 2 && visc) {
+            // ---- first half step on the tile + halo 2 from U^n on halo 4:
+            // (u, v, T) and tags on halo 4, stress and heat flux on halo 3,
+            // then the updated cells' (rho, u, v, p) staged as below
+            constexpr int AX = TX + 8, AY = TY + 8, BX = TX + 6, BY = TY + 6, BXY = BX * BY;
+            T* const a_u = (T*)smem;
+            T* const a_v = a_u + AX * AY;
+            T* const a_T = a_v + AX * AY;
+            T* const b_f = a_T + AX * AY;
+            uint8_t* const a_ct = (uint8_t*)(b_f + 5 * BXY);
+            const T hdt = (T)(dtd * 0.5);
+            for (int n = tid; n < AX * AY; n += NT) {
+                int li = n % AX, lj = n / AX;
+                int gi = min(max(i0 + li - 4, 0), nx - 1);
+                int gj = min(max(j0 + lj - 4, 0), ny - 1);
+                const unsigned g = (unsigned)(gj * nx + gi);
+                T u, v, p, tt;
+                primitives(k, in0[g], in1[g], in2[g], in3[g], in4[g], u, v, p, tt);
+                a_u[n] = u; a_v[n] = v; a_T[n] = tt; a_ct[n] = ctp[g];
+            }
+            __syncthreads();
+            for (int n = tid; n < BXY; n += NT) {
+                int ti = n % BX, tj = n / BX;
+                viscous_flux<T, AX>(k, a_u, a_v, a_T, a_ct, (tj + 1) * AX + (ti + 1), i0 + ti - 3, j0 + tj - 3, n,
+                                    b_f, b_f + BXY, b_f + 2 * BXY, b_f + 3 * BXY, b_f + 4 * BXY);
+            }
+            __syncthreads();
+            // updated cells in registers until the half step's arrays are dead
+            constexpr int SN = (SX * SY + NT - 1) / NT;
+            T hq[SN][4];
+            uint8_t hc[SN];
+#pragma unroll
+            for (int m = 0; m < SN; m++) {
+                const int n = tid + m * NT;
+                if (n >= SX * SY) break;
+                const int li = n % SX, lj = n / SX;
+                const int ci = i0 + li - 2, cj = j0 + lj - 2;
+                const int gi = min(max(ci, 0), nx - 1), gj = min(max(cj, 0), ny - 1);
+                const unsigned g = (unsigned)(gj * nx + gi);
+                const int sa = (lj + 2) * AX + (li + 2);
+                const uint8_t c = a_ct[sa];
+                T v[5] = {in0[g], in1[g], in2[g], in3[g], in4[g]};
+                const bool interior = !(c == CT_SOLID || c == CT_GHOST) && gi > 0 && gi < nx - 1 && gj > 0 && gj < ny - 1;
+                if (interior)
+                    viscous_update<T, AX, BX>(k, a_u, a_v, sa, b_f, b_f + BXY, b_f + 2 * BXY, b_f + 3 * BXY, b_f + 4 * BXY,
+                                              (lj + 1) * BX + (li + 1), hdt, v);
+                // the tile's own cells: the half-step result (rho unchanged:
+                // that set's density plane is U^n's)
+                if (li >= 2 && li < TX + 2 && lj >= 2 && lj < TY + 2 && ci < nx && cj < ny)
+                    for (int f = 1; f < 5; f++) Plane<T>(cd.b.U[r.blend][f]).put(g, v[f]);
+                T u, vv, p;
+                primitives_uvp(k, v[0], v[1], v[2], v[3], v[4], u, vv, p);
+                hq[m][0] = v[0]; hq[m][1] = u; hq[m][2] = vv; hq[m][3] = p;
+                hc[m] = c;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < SN; m++) {
+                const int n = tid + m * NT;
+                if (n >= SX * SY) break;
+                s_r[n] = hq[m][0]; s_u[n] = hq[m][1]; s_v[n] = hq[m][2]; s_p[n] = hq[m][3]; s_ct[n] = hc[m];
+            }
+        } else
         // ---- stage tile + halo 2: conserved -> (rho, u, v, p) in LDS
         for (int n = tid; n < SX * SY; n += NT) {
             int li = n % SX, lj = n / SX;
@@ -541,6 +705,10 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, KAHAN>())) void k_stage
         }
         __syncthreads();
 
+        // the state the update starts from: set r.in, or the fused half step's
+        // result stored above
+        const int vs = (MODE == 2 && visc) ? r.blend : r.in;
+        const Plane<T> st1(cd.b.U[vs][1]), st2(cd.b.U[vs][2]), st3(cd.b.U[vs][3]), st4(cd.b.U[vs][4]);
         MT vmax_local = MT(0);
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
@@ -556,7 +724,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, KAHAN>())) void k_stage
             const unsigned g = (unsigned)(gj * nx + gi);
             const int sc = (lj + 2) * SX + (li + 2);
             const uint8_t c = s_ct[sc];
-            T v0[5] = {s_r[sc], in1[g], in2[g], in3[g], in4[g]};
+            T v0[5] = {s_r[sc], st1[g], st2[g], st3[g], st4[g]};
             T nv[5];
             if (c == CT_SOLID || c == CT_GHOST) {
 #pragma unroll
@@ -607,7 +775,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, KAHAN>())) void k_stage
             }
             T out[5];
             T carry[5] = {T(0), T(0), T(0), T(0), T(0)};
-            if (MODE == 0) {
+            if (MODE != 1) {
 #pragma unroll
                 for (int f = 0; f < 5; f++) out[f] = nv[f];
             } else {
@@ -631,7 +799,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, KAHAN>())) void k_stage
                     for (int f = 0; f < 5; f++) Plane<float>(cd.b.K[f]).put(g, 0.0f);
                 continue;
             }
-            const bool apply_bc = (MODE == 0) || !visc;
+            const bool apply_bc = (MODE != 1) || !visc;
             if (apply_bc) {
                 if (c == CT_GHOST) {
                     if (KAHAN && MODE == 1)
@@ -939,38 +1107,10 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
         }
         __syncthreads();
 
-        const T i2x = (T)k.inv_2dx, i2y = (T)k.inv_2dy, ix = (T)k.inv_dx, iy = (T)k.inv_dy;
         for (int n = tid; n < QX * QY; n += NT) {
             int ti = n % QX, tj = n / QX;
-            int gi = i0 + ti - 1, gj = j0 + tj - 1;
-            int s = (tj + 1) * SX + (ti + 1);
-            if (gi < 0 || gi >= nx || gj < 0 || gj >= ny || s_ct[s] == CT_SOLID) {
-                s_txx[n] = T(0); s_tyy[n] = T(0); s_txy[n] = T(0); s_qx[n] = T(0); s_qy[n] = T(0);
-                continue;
-            }
-            T Tv = s_T[s];
-            T mu = sutherland(k, Tv);
-            T kk = mu * (T)k.cp * Num<T>::rcp((T)k.prandtl);
-            T dudx = (s_u[s + 1] - s_u[s - 1]) * i2x, dvdx = (s_v[s + 1] - s_v[s - 1]) * i2x;
-            T dudy = (s_u[s + SX] - s_u[s - SX]) * i2y, dvdy = (s_v[s + SX] - s_v[s - SX]) * i2y;
-            T dTdx = (s_T[s + 1] - s_T[s - 1]) * i2x, dTdy = (s_T[s + SX] - s_T[s - SX]) * i2y;
-            if (gi == 0) {
-                dudx = (s_u[s + 1] - s_u[s]) * ix; dvdx = (s_v[s + 1] - s_v[s]) * ix; dTdx = (s_T[s + 1] - s_T[s]) * ix;
-            } else if (gi == nx - 1) {
-                dudx = (s_u[s] - s_u[s - 1]) * ix; dvdx = (s_v[s] - s_v[s - 1]) * ix; dTdx = (s_T[s] - s_T[s - 1]) * ix;
-            }
-            if (gj == 0) {
-                dudy = (s_u[s + SX] - s_u[s]) * iy; dvdy = (s_v[s + SX] - s_v[s]) * iy; dTdy = (s_T[s + SX] - s_T[s]) * iy;
-            } else if (gj == ny - 1) {
-                dudy = (s_u[s] - s_u[s - SX]) * iy; dvdy = (s_v[s] - s_v[s - SX]) * iy; dTdy = (s_T[s] - s_T[s - SX]) * iy;
-            }
-            T divv = dudx + dvdy;
-            const T two3 = T(2) / T(3);
-            s_txx[n] = mu * (T(2) * dudx - two3 * divv);
-            s_tyy[n] = mu * (T(2) * dvdy - two3 * divv);
-            s_txy[n] = mu * (dudy + dvdx);
-            s_qx[n] = -kk * dTdx;
-            s_qy[n] = -kk * dTdy;
+            viscous_flux<T, SX>(k, s_u, s_v, s_T, s_ct, (tj + 1) * SX + (ti + 1), i0 + ti - 1, j0 + tj - 1, n,
+                                s_txx, s_tyy, s_txy, s_qx, s_qy);
         }
         __syncthreads();
 
@@ -990,27 +1130,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
             for (int f = 0; f < 5; f++) v[f] = in[f][g];
             T Tin = s_T[s];
             bool interior = !(c == CT_SOLID || c == CT_GHOST) && gi > 0 && gi < nx - 1 && gj > 0 && gj < ny - 1;
-            if (interior) {
-                T dtxx = (s_txx[t + 1] - s_txx[t - 1]) * i2x;
-                T dtxy_y = (s_txy[t + QX] - s_txy[t - QX]) * i2y;
-                T dtxy_x = (s_txy[t + 1] - s_txy[t - 1]) * i2x;
-                T dtyy = (s_tyy[t + QX] - s_tyy[t - QX]) * i2y;
-                T dqx = (s_qx[t + 1] - s_qx[t - 1]) * i2x;
-                T dqy = (s_qy[t + QX] - s_qy[t - QX]) * i2y;
-                T dudx = (s_u[s + 1] - s_u[s - 1]) * i2x, dudy = (s_u[s + SX] - s_u[s - SX]) * i2y;
-                T dvdx = (s_v[s + 1] - s_v[s - 1]) * i2x, dvdy = (s_v[s + SX] - s_v[s - SX]) * i2y;
-                T Phi = s_txx[t] * dudx + s_tyy[t] * dvdy + s_txy[t] * (dudy + dvdx);
-                T wxp = s_u[s + 1] * s_txx[t + 1] + s_v[s + 1] * s_txy[t + 1];
-                T wxm = s_u[s - 1] * s_txx[t - 1] + s_v[s - 1] * s_txy[t - 1];
-                T wyp = s_u[s + SX] * s_txy[t + QX] + s_v[s + SX] * s_tyy[t + QX];
-                T wym = s_u[s - SX] * s_txy[t - QX] + s_v[s - SX] * s_tyy[t - QX];
-                T rhsE = -dqx - dqy + (wxp - wxm) * i2x + (wyp - wym) * i2y;
-                T rhse = -dqx - dqy + Phi;
-                v[1] += hdt * (dtxx + dtxy_y);
-                v[2] += hdt * (dtxy_x + dtyy);
-                v[3] += hdt * rhsE;
-                v[4] += hdt * rhse;
-            }
+            if (interior) viscous_update<T, SX, QX>(k, s_u, s_v, s, s_txx, s_tyy, s_txy, s_qx, s_qy, t, hdt, v);
             if (FINAL) {
                 if (c == CT_GHOST) continue;  // filled by k_ghost
                 bc_pointwise(k, gi, gj, c, v);
@@ -1293,6 +1413,15 @@ struct Launch {
         if (kahan && sizeof(T) == 4) k_stage<T, TX, TY, 0, true><<<g, NT, 0, s>>>(c, w, n, ri, rv);
         else k_stage<T, TX, TY, 0, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
     }
+    // Stage 1 with the first viscous half step fused in (FP32, uncompensated).
+    static bool stage1_fused(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv, bool kahan) {
+        if constexpr (fused_d1_fits<T, TX, TY>()) {
+            if (kahan) return false;
+            k_stage<T, TX, TY, 2, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+            return true;
+        }
+        return false;
+    }
     static void stage2(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv, bool kahan) {
         if (kahan) k_stage<T, TX, TY, 1, true><<<g, NT, 0, s>>>(c, w, n, ri, rv);
         else k_stage<T, TX, TY, 1, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
@@ -1321,6 +1450,7 @@ static bool dispatch_tile(int tx, int ty, const KernelCall& kc);
     if (tx == A && ty == B) {                                                                      \
         switch (kc.op) {                                                                           \
         case OP_STAGE1: Launch<T, A, B>::stage1(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan); break; \
+        case OP_STAGE1_FUSED: if (!Launch<T, A, B>::stage1_fused(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan)) return false; break; \
         case OP_STAGE2: Launch<T, A, B>::stage2(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan); break; \
         case OP_DIFFUSE: Launch<T, A, B>::diffuse(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.rv, false); break; \
         case OP_DIFFUSE_FINAL: Launch<T, A, B>::diffuse(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.rv, true); break; \
@@ -1349,6 +1479,14 @@ bool dispatch_tile<double>(int tx, int ty, const KernelCall& kc) {
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc) {
     return fp64 ? dispatch_tile<double>(tx, ty, kc) : dispatch_tile<float>(tx, ty, kc);
+}
+
+bool fused_d1_supported(int tx, int ty, bool fp64) {
+    if (fp64) return false;
+#define FVM_FITS(A, B) if (tx == A && ty == B) return fused_d1_fits<float, A, B>();
+    FVM_TILE_TABLE(FVM_FITS)
+#undef FVM_FITS
+    return false;
 }
 
 bool tile_supported(int tx, int ty, bool fp64) {

@@ -6,7 +6,7 @@
 
 namespace fvm {
 
-enum KernelOp { OP_STAGE1, OP_STAGE2, OP_DIFFUSE, OP_DIFFUSE_FINAL, OP_BC_INPLACE, OP_MAXIMA };
+enum KernelOp { OP_STAGE1, OP_STAGE1_FUSED, OP_STAGE2, OP_DIFFUSE, OP_DIFFUSE_FINAL, OP_BC_INPLACE, OP_MAXIMA };
 
 struct KernelCall {
     KernelOp op;
@@ -21,6 +21,8 @@ struct KernelCall {
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc);
 bool tile_supported(int tx, int ty, bool fp64);
+// Whether stage 1 with the first viscous half step fused in is built for a tile.
+bool fused_d1_supported(int tx, int ty, bool fp64);
 // Tiles of one item of the per-tile-maxima reduction: case_list holds one
 // item (case, chunk) per REDUCE_CHUNK tiles of every active case.
 constexpr int REDUCE_CHUNK = 1024;

@@ -3,8 +3,11 @@
 Every stage kernel is compiled for a set of tile shapes (cells per workgroup
 tile, 256 threads).  The shapes trade halo recomputation (small tiles) against
 register pressure and workgroup-level imbalance (large tiles) differently for
-each grid size and for inviscid vs viscous cases.  This module times a step on
-each shape for a set of workloads, keeps the fastest per bucket, and writes
+each grid size and for inviscid vs viscous cases.  A viscous step also has two
+schedules: the first half step as a launch of its own, or fused into stage 1
+(recomputed on a wider halo: fewer launches, more arithmetic).  This module
+times a step on each shape (and schedule) for a set of workloads, keeps the
+fastest per bucket, and writes
 
 * tiles.json        -- the measured table (ms per step for every shape), and
 * tile_table.inc    -- the default choice per bucket, compiled into
@@ -40,9 +43,10 @@ WORKLOADS = [
 ]
 
 
-def time_plain(p: P.FvmParams, tile, steps: int, warmup: int = 10) -> float:
+def time_plain(p: P.FvmParams, tile, steps: int, warmup: int = 10, schedule: int = P.SCHEDULE_SPLIT) -> float:
     s = CFDSolver(0)
     s.set_tile(*tile)
+    s.set_viscous_schedule(schedule)
     q = p.copy()
     q.dt_mode = P.DT_DEVICE_CFL
     s.initialize(q)
@@ -57,9 +61,10 @@ def time_plain(p: P.FvmParams, tile, steps: int, warmup: int = 10) -> float:
     return dt * 1e3
 
 
-def time_batch(cases, tile, steps: int) -> float:
+def time_batch(cases, tile, steps: int, schedule: int = P.SCHEDULE_SPLIT) -> float:
     b = BatchEngine(0, mem_budget_bytes=16 << 30, max_cases=len(cases))
     b.set_tile(*tile)
+    b.set_viscous_schedule(schedule)
     for p in cases:
         b.add(p)
     b.prepare_graphs()
@@ -71,6 +76,13 @@ def time_batch(cases, tile, steps: int) -> float:
     dt = (time.perf_counter() - t0) / steps
     b.close()
     return dt * 1e3
+
+
+def parse_choice(key: str):
+    """'64x7' or '32x4+fused' -> (tile_x, tile_y, fused)."""
+    shape, _, sched = key.partition("+")
+    tx, ty = (int(x) for x in shape.split("x"))
+    return tx, ty, int(sched == "fused")
 
 
 def bucket_of(cells: int) -> int:
@@ -95,25 +107,33 @@ def main(argv=None):
         times = {}
         for t in TILES:
             times["%dx%d" % t] = time_plain(p, t, steps)
+            if p.enable_viscosity and CFDSolver.fused_schedule_supported(*t):
+                times["%dx%d+fused" % t] = time_plain(p, t, steps, schedule=P.SCHEDULE_FUSED)
         best = min(times, key=times.get)
         result["plain"][name] = {"cells": p.nx * p.ny, "viscous": bool(p.enable_viscosity), "ms_per_step": times,
                                  "best": best}
-        table[(int(p.enable_viscosity), bucket_of(p.nx * p.ny))] = tuple(int(x) for x in best.split("x"))
+        table[(int(p.enable_viscosity), bucket_of(p.nx * p.ny))] = parse_choice(best)
         print(f"{name}: best {best} {times[best]:.4f} ms/step (64x8: {times['64x8']:.4f})", flush=True)
     cases = P.mixed_batch(512)
-    times = {"%dx%d" % t: time_batch(cases, t, 40) for t in TILES}
+    times = {}
+    for t in TILES:
+        times["%dx%d" % t] = time_batch(cases, t, 40)
+        if CFDSolver.fused_schedule_supported(*t):
+            times["%dx%d+fused" % t] = time_batch(cases, t, 40, schedule=P.SCHEDULE_FUSED)
     best = min(times, key=times.get)
     result["batch"]["mixed_512"] = {"cells": sum(c.nx * c.ny for c in cases), "ms_per_step": times, "best": best}
     print(f"mixed_512 batch: best {best} {times[best]:.4f} ms/step", flush=True)
     with open(os.path.join(a.out, "tiles.json"), "w") as f:
         json.dump(result, f, indent=1)
-    bx, by = (int(x) for x in best.split("x"))
+    bx, by, bf = parse_choice(best)
     with open(os.path.join(a.out, "tile_table.inc"), "w") as f:
         f.write("// Generated by fvm_amd/autotune.py from measured step times on one MI355X.\n")
-        f.write("// entries {viscous, log4(cells / 32768) bucket, tile_x, tile_y}\n")
+        f.write("// entries {viscous, log4(cells / 32768) bucket, tile_x, tile_y, "
+                "first viscous half step fused into stage 1}\n")
         f.write("#define FVM_TILE_ENTRIES " + ", ".join(
-            f"{{{v}, {bk}, {tx}, {ty}}}" for (v, bk), (tx, ty) in sorted(table.items())) + "\n")
-        f.write(f"// batched engine default\n#define FVM_BATCH_TILE_X {bx}\n#define FVM_BATCH_TILE_Y {by}\n")
+            f"{{{v}, {bk}, {tx}, {ty}, {fu}}}" for (v, bk), (tx, ty, fu) in sorted(table.items())) + "\n")
+        f.write(f"// batched engine default\n#define FVM_BATCH_TILE_X {bx}\n#define FVM_BATCH_TILE_Y {by}\n"
+                f"#define FVM_BATCH_FUSED {bf}\n")
     return result
 
 

@@ -114,5 +114,8 @@ class BatchEngine:
     def set_tile(self, tx: int, ty: int):
         check(lib().fvm_batch_set_tile(self._h, tx, ty), "fvm_batch_set_tile")
 
+    def set_viscous_schedule(self, schedule: int):
+        check(lib().fvm_batch_set_viscous_schedule(self._h, int(schedule)), "fvm_batch_set_viscous_schedule")
+
     def use_graph(self, on: bool):
         check(lib().fvm_batch_use_graph(self._h, 1 if on else 0), "fvm_batch_use_graph")

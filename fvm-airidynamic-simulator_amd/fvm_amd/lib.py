@@ -74,6 +74,8 @@ SIGNATURES = [
     ("fvm_synchronize", _i, [_vp]),
     ("fvm_get_size", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     ("fvm_set_tile", _i, [_vp, _i, _i]),
+    ("fvm_set_viscous_schedule", _i, [_vp, _i]),
+    ("fvm_fused_schedule_supported", _i, [_i, _i, _i, ctypes.POINTER(_i)]),
     ("fvm_enable_timing", _i, [_vp, _i]),
     ("fvm_kernel_timing", _i, [_vp, _i, ctypes.c_char_p, _i, _dp, _i64p]),
     ("fvm_batch_create", _i, [_i, ctypes.c_size_t, _i, ctypes.POINTER(_vp)]),
@@ -93,6 +95,7 @@ SIGNATURES = [
     ("fvm_batch_bytes_in_use", _i, [_vp, _szp]),
     ("fvm_batch_synchronize", _i, [_vp]),
     ("fvm_batch_set_tile", _i, [_vp, _i, _i]),
+    ("fvm_batch_set_viscous_schedule", _i, [_vp, _i]),
     ("fvm_batch_use_graph", _i, [_vp, _i]),
 ]
 

@@ -23,6 +23,7 @@ LIMITER_MINMOD, LIMITER_VAN_LEER = 0, 1
 WALL_REFERENCE, WALL_SLIP = 0, 1
 PRECISION_FP32, PRECISION_FP32_KAHAN, PRECISION_FP64 = 0, 1, 2
 DT_HOST, DT_DEVICE_CFL = 0, 1
+SCHEDULE_AUTO, SCHEDULE_SPLIT, SCHEDULE_FUSED = 0, 1, 2
 NOISE_NONE, NOISE_DENSITY, NOISE_TRANSVERSE_VELOCITY = 0, 1, 2
 FIELD_TEMPERATURE, FIELD_PRESSURE, FIELD_DENSITY, FIELD_VELOCITY_MAG, FIELD_MACH, FIELD_CELL_TYPE = range(6)
 

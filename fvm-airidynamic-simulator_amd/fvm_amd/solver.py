@@ -189,6 +189,18 @@ class CFDSolver:
     def set_tile(self, tx: int, ty: int):
         check(lib().fvm_set_tile(self._h, tx, ty), "fvm_set_tile")
 
+    def set_viscous_schedule(self, schedule: int):
+        """P.SCHEDULE_AUTO (default), P.SCHEDULE_SPLIT or P.SCHEDULE_FUSED (the
+        first viscous half step inside stage 1); all give bitwise the same state."""
+        check(lib().fvm_set_viscous_schedule(self._h, int(schedule)), "fvm_set_viscous_schedule")
+
+    @staticmethod
+    def fused_schedule_supported(tile_x: int, tile_y: int, precision: int = P.PRECISION_FP32) -> bool:
+        v = ctypes.c_int()
+        check(lib().fvm_fused_schedule_supported(tile_x, tile_y, precision, ctypes.byref(v)),
+              "fvm_fused_schedule_supported")
+        return bool(v.value)
+
     def enable_timing(self, on: bool = True):
         check(lib().fvm_enable_timing(self._h, 1 if on else 0), "fvm_enable_timing")
 

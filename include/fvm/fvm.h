@@ -286,6 +286,20 @@ int fvm_get_size(fvm_solver* s, int* nx, int* ny);
  * (64x4, 64x6, 64x7, 64x8, 64x12, 64x16, 32x4, 32x8, 32x16, 128x4, 128x8, 96x8;
  * FP64: 64x8, 32x8); 0,0 = the autotuned default for the grid. */
 int fvm_set_tile(fvm_solver* s, int tile_x, int tile_y);
+/* Schedule of a viscous step (no reference counterpart: the reference always
+ * launches its diffusion kernel, solver.cu:3299-3304 and 3343-3354, as its own
+ * pass).  FVM_SCHEDULE_SPLIT: the first half step is a launch of its own.
+ * FVM_SCHEDULE_FUSED: it is recomputed inside stage 1 on a wider halo (FP32
+ * only; tiles whose LDS does not fit, FP64 and the compensated variant keep
+ * the split schedule).  FVM_SCHEDULE_AUTO (default): the autotuned choice for
+ * the grid, with the tile of fvm_set_tile(0, 0).  All give bitwise the same
+ * state. */
+#define FVM_SCHEDULE_AUTO 0
+#define FVM_SCHEDULE_SPLIT 1
+#define FVM_SCHEDULE_FUSED 2
+int fvm_set_viscous_schedule(fvm_solver* s, int schedule);
+/* Whether FVM_SCHEDULE_FUSED is built for a tile shape and precision. */
+int fvm_fused_schedule_supported(int tile_x, int tile_y, int precision, int* supported);
 /* Name and average duration of each kernel launched by the last fvm_advance
  * call when timing is enabled (HIP events on the handle's stream). */
 int fvm_enable_timing(fvm_solver* s, int enable);
@@ -332,6 +346,9 @@ int fvm_batch_bytes_in_use(fvm_batch* b, size_t* bytes);
 int fvm_batch_synchronize(fvm_batch* b);
 /* Tile shape of the batched kernels (0,0 = the autotuned default). */
 int fvm_batch_set_tile(fvm_batch* b, int tile_x, int tile_y);
+/* fvm_set_viscous_schedule for the batched engine (AUTO: the autotuned batch
+ * default). */
+int fvm_batch_set_viscous_schedule(fvm_batch* b, int schedule);
 /* Use a captured HIP graph for fvm_batch_advance (default on). */
 int fvm_batch_use_graph(fvm_batch* b, int enable);
 

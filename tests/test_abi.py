@@ -185,13 +185,33 @@ def test_tuned_tile_table_uses_compiled_shapes():
     compiled = {tuple(int(v) for v in x) for x in re.findall(r"X\((\d+),\s*(\d+)\)", m.group(1))}
     inc = open(os.path.join(csrc, "tile_table.inc")).read()
     entries = [tuple(int(v) for v in e) for e in
-               re.findall(r"\{(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\}", re.search(r"#define FVM_TILE_ENTRIES (.*)", inc).group(1))]
+               re.findall(r"\{(\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\}",
+                          re.search(r"#define FVM_TILE_ENTRIES (.*)", inc).group(1))]
     assert entries
-    for visc, bucket, tx, ty in entries:
-        assert visc in (0, 1) and 0 <= bucket <= 4 and (tx, ty) in compiled
+    from fvm_amd.solver import CFDSolver
+    for visc, bucket, tx, ty, fused in entries:
+        assert visc in (0, 1) and 0 <= bucket <= 4 and (tx, ty) in compiled and fused in (0, 1)
+        # the fused schedule only where it is built, and only for viscous buckets
+        assert not fused or (visc and CFDSolver.fused_schedule_supported(tx, ty))
     bx = int(re.search(r"#define FVM_BATCH_TILE_X (\d+)", inc).group(1))
     by = int(re.search(r"#define FVM_BATCH_TILE_Y (\d+)", inc).group(1))
-    assert (bx, by) in compiled
-    from fvm_amd.autotune import bucket_of
+    bf = int(re.search(r"#define FVM_BATCH_FUSED (\d+)", inc).group(1))
+    assert (bx, by) in compiled and (not bf or CFDSolver.fused_schedule_supported(bx, by))
+    from fvm_amd.autotune import bucket_of, parse_choice
+    assert parse_choice("64x7") == (64, 7, 0) and parse_choice("32x4+fused") == (32, 4, 1)
     assert [bucket_of(32768 * 4 ** k) for k in range(5)] == [0, 1, 2, 3, 4]
     assert bucket_of(256 * 256) == 1 and bucket_of(64) == 0 and bucket_of(1 << 30) == 4   # ties round up
+
+
+def test_fused_schedule_support_query():
+    """The fused viscous schedule is built for the FP32 tiles whose LDS fits a
+    workgroup (not 128x8: 66 KB), never for FP64 or the compensated variant;
+    the query needs no GPU."""
+    from fvm_amd import params as P
+    from fvm_amd.solver import CFDSolver
+    for t in ((64, 7), (32, 4), (32, 8), (32, 16), (64, 16), (96, 8), (128, 4)):
+        assert CFDSolver.fused_schedule_supported(*t), t
+    assert not CFDSolver.fused_schedule_supported(128, 8)
+    assert not CFDSolver.fused_schedule_supported(64, 8, P.PRECISION_FP64)
+    assert not CFDSolver.fused_schedule_supported(64, 7, P.PRECISION_FP32_KAHAN)
+    assert not CFDSolver.fused_schedule_supported(5, 5)
