@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--overload-seconds", type=float, default=4.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the 256x128 Euler case and the CPU mix subset")
+    ap.add_argument("--direct", action="store_true",
+                    help="launch every kernel directly instead of replaying captured graphs (kernel tracing)")
     return ap.parse_args()
 
 
@@ -181,7 +183,7 @@ def other_configs(with_cpu):
     return out
 
 
-def batched_vs_plain(ncases, nsteps):
+def batched_vs_plain(ncases, nsteps, graphs=True):
     """The mixed batch (grids 64x64..512x256, five shapes, Mach 0.3-4, half
     viscous, per-case CFL dt): all cases in the batched engine vs the same
     cases one at a time through the plain solver, nsteps each, same GPU."""
@@ -193,9 +195,11 @@ def batched_vs_plain(ncases, nsteps):
     # both sides capture their step graphs and take 2 warm-up steps before
     # their timed windows, so no capture is timed on either side
     b = BatchEngine(0, mem_budget_bytes=16 << 30, max_cases=ncases)
+    b.use_graph(graphs)
     for p in cases:
         b.add(p)
-    b.prepare_graphs()
+    if graphs:
+        b.prepare_graphs()
     b.advance(2)
     b.synchronize()
     t0 = time.perf_counter()
@@ -204,11 +208,13 @@ def batched_vs_plain(ncases, nsteps):
     tb = time.perf_counter() - t0
     b.close()
     s = fvm_amd.CFDSolver(0)
+    s.use_graph(graphs)
     tp = 0.0
     for p in cases:
         s.initialize(p)
         q = p.copy()
-        s.prepare_graphs(q)
+        if graphs:
+            s.prepare_graphs(q)
         s.advance(q, 2)
         s.synchronize()
         t0 = time.perf_counter()
@@ -250,11 +256,13 @@ def main():
                            center=(512.0, 512.0) if (a.nx, a.ny) == (2048, 1024) else None)
     p.dt_mode = P.DT_DEVICE_CFL
     s = fvm_amd.CFDSolver(0)
+    s.use_graph(not a.direct)
     s.initialize(p)
     q = p.copy()
     # the 2- and 16-step graphs the timed steps replay are captured here, so
     # whatever --warmup is, the timed window holds no graph capture
-    s.prepare_graphs(q)
+    if not a.direct:
+        s.prepare_graphs(q)
     s.advance(q, a.warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -290,8 +298,8 @@ def main():
         roofline["valu"] = {"achieved": lane_ops / 1e12, "peak": VALU_PEAK / 1e12, "unit": "lane-Tops/s",
                             "frac": lane_ops / VALU_PEAK, "insts_per_launch": valu}
         roofline["note"] = ("the stage kernels are vector-ALU issue bound: valu.frac is their VALU issue "
-                            "fraction at the measured duration; HBM traffic is within ~10% of the "
-                            "algorithmic bytes")
+                            "fraction at the measured duration"
+                            + (f"; HBM traffic is {traffic / alg_bytes:.2f}x the algorithmic bytes" if traffic else ""))
 
     t_sim, nsteps_dev = s.time()
     out = {
@@ -309,7 +317,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"viscous Navier-Stokes cylinder {p.nx}x{p.ny}, Re 200, Mach 0.3, HLLC + van Leer, "
                                "SSP-RK2 + Strang viscous half steps, CFL 0.4 device dt",
-                   "nx": p.nx, "ny": p.ny, "parallelism": "single GPU"},
+                   "nx": p.nx, "ny": p.ny, "parallelism": "single GPU",
+                   "launch": "direct" if a.direct else "hip graph"},
         "hbm_peak_fraction_at_102B": value * B_ALG_PER_CELL_UPDATE / HBM_PEAK,
         "roofline": roofline,
         "kernel_ms": {k: v[0] for k, v in kt.items()},
@@ -317,7 +326,7 @@ def main():
     }
     s.close()
     if not a.no_batch:
-        out["batched"] = batched_vs_plain(a.batch_cases, a.batch_steps)
+        out["batched"] = batched_vs_plain(a.batch_cases, a.batch_steps, graphs=not a.direct)
     if not a.no_serve:
         out["served"] = served(a.serve_rate, a.serve_seconds, a.serve_step_scale)
         # above capacity: the queue grows, latency is queueing; the run stops at 6x the window

@@ -30,6 +30,7 @@ struct fvm_solver {
     int tile_x = 0, tile_y = 0;
     int schedule = FVM_SCHEDULE_AUTO;
     bool timing = false;
+    bool use_graph = true;
 };
 
 struct fvm_batch {
@@ -292,7 +293,7 @@ int fvm_advance(fvm_solver* s, fvm_params* p, int nsteps) {
     if (nsteps < 0) return set_error(FVM_ERR_INVALID_ARGUMENT, "negative step count");
     int rc = apply_params(s, *p);
     if (rc) return rc;
-    rc = s->eng->advance(nsteps, p->dt_mode == FVM_DT_DEVICE_CFL);
+    rc = s->eng->advance(nsteps, s->use_graph && p->dt_mode == FVM_DT_DEVICE_CFL);
     if (rc) return rc;
     if (p->dt_mode == FVM_DT_DEVICE_CFL) {
         double t = 0;
@@ -506,6 +507,12 @@ int fvm_set_viscous_schedule(fvm_solver* s, int schedule) {
         return set_error(FVM_ERR_INVALID_ARGUMENT, "unknown viscous schedule");
     s->schedule = schedule;
     if (s->eng) s->eng->set_fused_d1(schedule_fused(s, s->p));
+    return FVM_OK;
+}
+
+int fvm_use_graph(fvm_solver* s, int enable) {
+    if (!s) return set_error(FVM_ERR_INVALID_ARGUMENT, "null solver handle");
+    s->use_graph = enable != 0;
     return FVM_OK;
 }
 

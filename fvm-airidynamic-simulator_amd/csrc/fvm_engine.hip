@@ -852,10 +852,11 @@ int Engine::flush_timed_set(int set) {
     TimedSet& ts = tset_[set];
     if (!ts.pending) return FVM_OK;
     ts.pending = false;
-    FVM_HIP(hipEventSynchronize(ts.ev[ts.n]));
+    if (ts.n == 0) return FVM_OK;
+    FVM_HIP(hipEventSynchronize(ts.ev[2 * ts.n - 1]));
     for (int i = 0; i < ts.n; i++) {
         float ms = 0;
-        FVM_HIP(hipEventElapsedTime(&ms, ts.ev[i], ts.ev[i + 1]));
+        FVM_HIP(hipEventElapsedTime(&ms, ts.ev[2 * i], ts.ev[2 * i + 1]));
         bool found = false;
         for (auto& kv : timing_list_)
             if (kv.first == ts.name[i]) {
@@ -886,23 +887,31 @@ void Engine::clear_timings() {
 }
 
 // Opens the event set of one step: the set used two steps ago is read first.
+// Each timed kernel is launched with a start / stop event pair on its own
+// dispatch, so the interval is the kernel's execution, as in a kernel trace.
 int Engine::timed_begin() {
     if (!timing_) return FVM_OK;
     int rc = flush_timed_set(tcur_);
     if (rc) return rc;
-    TimedSet& ts = tset_[tcur_];
-    ts.n = 0;
-    FVM_HIP(hipEventRecord(ts.ev[0], stream_));
+    tset_[tcur_].n = 0;
     return FVM_OK;
 }
 
-// Closes the interval of the kernel just launched.
+void Engine::timed_events(hipEvent_t* ev0, hipEvent_t* ev1) {
+    *ev0 = *ev1 = nullptr;
+    if (!timing_) return;
+    TimedSet& ts = tset_[tcur_];
+    if (ts.n >= kTimedMax) return;
+    *ev0 = ts.ev[2 * ts.n];
+    *ev1 = ts.ev[2 * ts.n + 1];
+}
+
+// Names the interval of the kernel just launched with timed_events().
 int Engine::timed(const char* name, int rc_before) {
     if (!timing_ || rc_before) return rc_before;
     TimedSet& ts = tset_[tcur_];
     if (ts.n >= kTimedMax) return set_error(FVM_ERR_INVALID_ARGUMENT, "too many timed launches in one step");
     ts.name[ts.n] = name;
-    FVM_HIP(hipEventRecord(ts.ev[ts.n + 1], stream_));
     ts.n++;
     return FVM_OK;
 }
@@ -936,6 +945,7 @@ int Engine::enqueue_step(int ph) {
         kc.count = d_counts_ + 1;
         kc.ri = v_k0;
         kc.rv = v_k0;
+        timed_events(&kc.ev0, &kc.ev1);
         launch_tiled(fp64_, tx_, ty_, kc);
         rc = timed("diffuse_half1", rc);
     }
@@ -945,11 +955,13 @@ int Engine::enqueue_step(int ph) {
     kc.count = d_counts_ + 0;
     kc.ri = i_s1;
     kc.rv = fused ? v_f1 : v_s1;
+    timed_events(&kc.ev0, &kc.ev1);
     launch_tiled(fp64_, tx_, ty_, kc);
     rc = timed(fused ? "stage1_fused" : "stage1", rc);
     kc.op = OP_STAGE2;
     kc.ri = i_s2;
     kc.rv = v_s2;
+    timed_events(&kc.ev0, &kc.ev1);
     launch_tiled(fp64_, tx_, ty_, kc);
     rc = timed("stage2", rc);
     if (any_visc_) {
@@ -959,11 +971,14 @@ int Engine::enqueue_step(int ph) {
         kc.count = d_counts_ + 1;
         kc.ri = v_k3;
         kc.rv = v_k3;
+        timed_events(&kc.ev0, &kc.ev1);
         launch_tiled(fp64_, tx_, ty_, kc);
         rc = timed("diffuse_half2", rc);
     }
+    hipEvent_t g0, g1;
+    timed_events(&g0, &g1);
     launch_ghost(fp64_, dt64(), true, dim3(grid_ghosts_ + grid_reduce_), stream_, d_cases_, d_ghosts_, d_counts_ + 2, i_g2,
-                 v_g2, grid_ghosts_, d_case_list_, d_counts_ + 3, single_case_);
+                 v_g2, grid_ghosts_, d_case_list_, d_counts_ + 3, single_case_, g0, g1);
     rc = timed("ghost2", rc);
     if (rc) return rc;
     if (timing_) {

@@ -167,6 +167,8 @@ private:
     void dev_free(void* p);
     int upload_items(const std::vector<WorkItem>& v, WorkItem** buf, size_t* cap, int* d_count, int count_slot);
     int timed_begin();
+    // Start / stop events for the next timed launch (null when not timing).
+    void timed_events(hipEvent_t* ev0, hipEvent_t* ev1);
     int timed(const char* name, int rc_before);
     int flush_timed_set(int set);
 
@@ -217,7 +219,7 @@ private:
     bool timing_ = false;
     static constexpr int kTimedMax = 8;
     struct TimedSet {
-        hipEvent_t ev[kTimedMax + 1] = {};
+        hipEvent_t ev[2 * kTimedMax] = {};   // start / stop of each timed launch
         const char* name[kTimedMax] = {};
         int n = 0;
         bool pending = false;

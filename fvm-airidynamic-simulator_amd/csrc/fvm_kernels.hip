@@ -16,6 +16,8 @@
 #include "fvm_device.hpp"
 #include "fvm_kernels.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -1405,37 +1407,43 @@ __global__ __launch_bounds__(64) void k_cons_final(const double* partial, int nb
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
+// A launch, with start / stop events on the dispatch itself when the engine
+// times its kernels (hipExtLaunchKernelGGL: the events bracket the kernel, not
+// the queue gaps around it, as a kernel trace does).
+template <typename F, typename... A>
+static void go(F kernel, const KernelCall& kc, A... args) {
+    if (kc.ev0) hipExtLaunchKernelGGL(kernel, kc.grid, dim3(NT), 0, kc.stream, kc.ev0, kc.ev1, 0, args...);
+    else kernel<<<kc.grid, NT, 0, kc.stream>>>(args...);
+}
+
 template <class T, int TX, int TY>
 struct Launch {
+    using CD = const CaseDesc<T>*;
     // Stage 1 of the compensated variant is the KAHAN instantiation only for
     // its FP64 dt (the compensation itself is in the blend of stage 2).
-    static void stage1(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv, bool kahan) {
-        if (kahan && sizeof(T) == 4) k_stage<T, TX, TY, 0, true><<<g, NT, 0, s>>>(c, w, n, ri, rv);
-        else k_stage<T, TX, TY, 0, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+    static void stage1(const KernelCall& kc) {
+        if (kc.kahan && sizeof(T) == 4) go(k_stage<T, TX, TY, 0, true>, kc, (CD)kc.cases, kc.work, kc.count, kc.ri, kc.rv);
+        else go(k_stage<T, TX, TY, 0, false>, kc, (CD)kc.cases, kc.work, kc.count, kc.ri, kc.rv);
     }
     // Stage 1 with the first viscous half step fused in (FP32, uncompensated).
-    static bool stage1_fused(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv, bool kahan) {
+    static bool stage1_fused(const KernelCall& kc) {
         if constexpr (fused_d1_fits<T, TX, TY>()) {
-            if (kahan) return false;
-            k_stage<T, TX, TY, 2, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+            if (kc.kahan) return false;
+            go(k_stage<T, TX, TY, 2, false>, kc, (CD)kc.cases, kc.work, kc.count, kc.ri, kc.rv);
             return true;
         }
         return false;
     }
-    static void stage2(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles ri, Roles rv, bool kahan) {
-        if (kahan) k_stage<T, TX, TY, 1, true><<<g, NT, 0, s>>>(c, w, n, ri, rv);
-        else k_stage<T, TX, TY, 1, false><<<g, NT, 0, s>>>(c, w, n, ri, rv);
+    static void stage2(const KernelCall& kc) {
+        if (kc.kahan) go(k_stage<T, TX, TY, 1, true>, kc, (CD)kc.cases, kc.work, kc.count, kc.ri, kc.rv);
+        else go(k_stage<T, TX, TY, 1, false>, kc, (CD)kc.cases, kc.work, kc.count, kc.ri, kc.rv);
     }
-    static void diffuse(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles r, bool fin) {
-        if (fin) k_diffuse<T, TX, TY, true><<<g, NT, 0, s>>>(c, w, n, r);
-        else k_diffuse<T, TX, TY, false><<<g, NT, 0, s>>>(c, w, n, r);
+    static void diffuse(const KernelCall& kc, bool fin) {
+        if (fin) go(k_diffuse<T, TX, TY, true>, kc, (CD)kc.cases, kc.work, kc.count, kc.rv);
+        else go(k_diffuse<T, TX, TY, false>, kc, (CD)kc.cases, kc.work, kc.count, kc.rv);
     }
-    static void bc_inplace(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles r) {
-        k_bc_inplace<T, TX, TY><<<g, NT, 0, s>>>(c, w, n, r);
-    }
-    static void maxima(dim3 g, hipStream_t s, const CaseDesc<T>* c, const WorkItem* w, const int* n, Roles r) {
-        k_maxima<T, TX, TY><<<g, NT, 0, s>>>(c, w, n, r);
-    }
+    static void bc_inplace(const KernelCall& kc) { go(k_bc_inplace<T, TX, TY>, kc, (CD)kc.cases, kc.work, kc.count, kc.ri); }
+    static void maxima(const KernelCall& kc) { go(k_maxima<T, TX, TY>, kc, (CD)kc.cases, kc.work, kc.count, kc.ri); }
 };
 
 // Tile shapes compiled for the FP32 kernels (the autotuner chooses among them)
@@ -1449,13 +1457,13 @@ static bool dispatch_tile(int tx, int ty, const KernelCall& kc);
 #define FVM_TRY(A, B)                                                                              \
     if (tx == A && ty == B) {                                                                      \
         switch (kc.op) {                                                                           \
-        case OP_STAGE1: Launch<T, A, B>::stage1(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan); break; \
-        case OP_STAGE1_FUSED: if (!Launch<T, A, B>::stage1_fused(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan)) return false; break; \
-        case OP_STAGE2: Launch<T, A, B>::stage2(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri, kc.rv, kc.kahan); break; \
-        case OP_DIFFUSE: Launch<T, A, B>::diffuse(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.rv, false); break; \
-        case OP_DIFFUSE_FINAL: Launch<T, A, B>::diffuse(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.rv, true); break; \
-        case OP_BC_INPLACE: Launch<T, A, B>::bc_inplace(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri); break; \
-        case OP_MAXIMA: Launch<T, A, B>::maxima(kc.grid, kc.stream, (const CaseDesc<T>*)kc.cases, kc.work, kc.count, kc.ri); break; \
+        case OP_STAGE1: Launch<T, A, B>::stage1(kc); break;                                        \
+        case OP_STAGE1_FUSED: if (!Launch<T, A, B>::stage1_fused(kc)) return false; break;         \
+        case OP_STAGE2: Launch<T, A, B>::stage2(kc); break;                                        \
+        case OP_DIFFUSE: Launch<T, A, B>::diffuse(kc, false); break;                               \
+        case OP_DIFFUSE_FINAL: Launch<T, A, B>::diffuse(kc, true); break;                          \
+        case OP_BC_INPLACE: Launch<T, A, B>::bc_inplace(kc); break;                                \
+        case OP_MAXIMA: Launch<T, A, B>::maxima(kc); break;                                        \
         default: return false;                                                                     \
         }                                                                                          \
         return true;                                                                               \
@@ -1502,19 +1510,24 @@ bool tile_supported(int tx, int ty, bool fp64) {
 
 void launch_ghost(bool fp64, bool dt64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
                   const int* n, Roles ri, Roles rv, int gblocks, const WorkItem* case_list, const int* ncases,
-                  int single_case) {
+                  int single_case, hipEvent_t ev0, hipEvent_t ev1) {
     if (!final_) gblocks = (int)grid.x;
     const CaseDesc<double>* c64 = (const CaseDesc<double>*)c;
     const CaseDesc<float>* c32 = (const CaseDesc<float>*)c;
+    KernelCall kc{};
+    kc.grid = grid;
+    kc.stream = s;
+    kc.ev0 = ev0;
+    kc.ev1 = ev1;
     if (fp64) {
-        if (final_) k_ghost<double, true, true><<<grid, NT, 0, s>>>(c64, items, n, ri, rv, gblocks, case_list, ncases, single_case);
-        else k_ghost<double, false, true><<<grid, NT, 0, s>>>(c64, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        if (final_) go(k_ghost<double, true, true>, kc, c64, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        else go(k_ghost<double, false, true>, kc, c64, items, n, ri, rv, gblocks, case_list, ncases, single_case);
     } else if (dt64) {
-        if (final_) k_ghost<float, true, true><<<grid, NT, 0, s>>>(c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
-        else k_ghost<float, false, true><<<grid, NT, 0, s>>>(c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        if (final_) go(k_ghost<float, true, true>, kc, c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        else go(k_ghost<float, false, true>, kc, c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
     } else {
-        if (final_) k_ghost<float, true, false><<<grid, NT, 0, s>>>(c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
-        else k_ghost<float, false, false><<<grid, NT, 0, s>>>(c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        if (final_) go(k_ghost<float, true, false>, kc, c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
+        else go(k_ghost<float, false, false>, kc, c32, items, n, ri, rv, gblocks, case_list, ncases, single_case);
     }
 }
 

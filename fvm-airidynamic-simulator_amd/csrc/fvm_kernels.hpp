@@ -17,6 +17,8 @@ struct KernelCall {
     const int* count;
     Roles ri, rv;
     bool kahan;
+    // non-null: start / stop events recorded on the dispatch (timing mode)
+    hipEvent_t ev0, ev1;
 };
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc);
@@ -34,7 +36,8 @@ constexpr int REDUCE_CHUNK = 1024;
 // dt64: the engine keeps its CFL maxima in FP64 (FP64 state, compensated variant).
 void launch_ghost(bool fp64, bool dt64, bool final_, dim3 grid, hipStream_t s, const void* c, const WorkItem* items,
                   const int* n, Roles ri, Roles rv, int gblocks = 0, const WorkItem* case_list = nullptr,
-                  const int* ncases = nullptr, int single_case = -1);
+                  const int* ncases = nullptr, int single_case = -1, hipEvent_t ev0 = nullptr,
+                  hipEvent_t ev1 = nullptr);
 void launch_ghost_prep(bool fp64, hipStream_t s, const void* c, const WorkItem* items, int n);
 void launch_mark(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int fixup, int fix_set);
 void launch_init(bool fp64, hipStream_t s, const void* c, int case_id, size_t ncells, int set_a, int set_b);

@@ -75,6 +75,7 @@ SIGNATURES = [
     ("fvm_get_size", _i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     ("fvm_set_tile", _i, [_vp, _i, _i]),
     ("fvm_set_viscous_schedule", _i, [_vp, _i]),
+    ("fvm_use_graph", _i, [_vp, _i]),
     ("fvm_fused_schedule_supported", _i, [_i, _i, _i, ctypes.POINTER(_i)]),
     ("fvm_enable_timing", _i, [_vp, _i]),
     ("fvm_kernel_timing", _i, [_vp, _i, ctypes.c_char_p, _i, _dp, _i64p]),

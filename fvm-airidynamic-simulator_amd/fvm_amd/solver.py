@@ -194,6 +194,10 @@ class CFDSolver:
         first viscous half step inside stage 1); all give bitwise the same state."""
         check(lib().fvm_set_viscous_schedule(self._h, int(schedule)), "fvm_set_viscous_schedule")
 
+    def use_graph(self, on: bool):
+        """Replay captured graphs in advance() (default) or launch directly."""
+        check(lib().fvm_use_graph(self._h, 1 if on else 0), "fvm_use_graph")
+
     @staticmethod
     def fused_schedule_supported(tile_x: int, tile_y: int, precision: int = P.PRECISION_FP32) -> bool:
         v = ctypes.c_int()

@@ -298,6 +298,10 @@ int fvm_set_tile(fvm_solver* s, int tile_x, int tile_y);
 #define FVM_SCHEDULE_SPLIT 1
 #define FVM_SCHEDULE_FUSED 2
 int fvm_set_viscous_schedule(fvm_solver* s, int schedule);
+/* Replay captured HIP graphs in fvm_advance with device dt (default on; off:
+ * every kernel is launched directly, e.g. for kernel-tracing tools).  No
+ * reference counterpart. */
+int fvm_use_graph(fvm_solver* s, int enable);
 /* Whether FVM_SCHEDULE_FUSED is built for a tile shape and precision. */
 int fvm_fused_schedule_supported(int tile_x, int tile_y, int precision, int* supported);
 /* Name and average duration of each kernel launched by the last fvm_advance
