@@ -565,6 +565,7 @@ int fvm_batch_create(int device, size_t mem_budget_bytes, int max_cases, fvm_bat
     rc = b->eng->set_tile(FVM_BATCH_TILE_X, FVM_BATCH_TILE_Y);
     if (rc) return rc;
     b->eng->set_fused_d1(batch_fused(b.get()));
+    b->eng->set_side_admission(true);
     rc = b->eng->reserve_cases(max_cases);
     if (rc) return rc;
     rc = b->eng->set_arena(mem_budget_bytes);
@@ -606,6 +607,8 @@ int fvm_batch_advance(fvm_batch* b, int32_t nsteps) {
     if (!b) return set_error(FVM_ERR_INVALID_ARGUMENT, "null batch");
     if (nsteps < 0) return set_error(FVM_ERR_INVALID_ARGUMENT, "negative step count");
     Engine& e = *b->eng;
+    int rc0 = e.throttle_begin();
+    if (rc0) return rc0;
     int left = nsteps;
     while (left > 0 && e.active_count() > 0) {
         // run until the next case completes (or nsteps are done)
@@ -621,7 +624,7 @@ int fvm_batch_advance(fvm_batch* b, int32_t nsteps) {
         left -= (int)chunk;
         e.retire_finished();
     }
-    return FVM_OK;
+    return e.throttle_end();
 }
 
 int fvm_batch_prepare_graphs(fvm_batch* b) {

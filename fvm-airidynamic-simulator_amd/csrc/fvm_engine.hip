@@ -90,9 +90,11 @@ Engine::Engine(int device, bool fp64, bool kahan) : device_(device), fp64_(fp64)
 Engine::~Engine() {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (side_) (void)hipStreamSynchronize(side_);
     drop_graphs();
     for (HostCase* c : cases_) {
         if (!c) continue;
+        if (c->ev_frozen) (void)hipEventDestroy(c->ev_frozen);
         if (use_arena_) {
             arena_.release(c->mem);
             arena_.release(c->ghost_mem);
@@ -116,13 +118,24 @@ Engine::~Engine() {
         for (auto& ev : ts.ev)
             if (ev) (void)hipEventDestroy(ev);
     if (ev_null_) (void)hipEventDestroy(ev_null_);
+    for (hipEvent_t e : ev_ring_)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& kv : staging_) {
+        if (kv.second.host) (void)hipHostFree(kv.second.host);
+        if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+    }
+    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 int Engine::init_stream() {
     FVM_HIP(hipSetDevice(device_));
     if (!stream_) FVM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (!main_) main_ = stream_;
+    if (!side_) FVM_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     if (!ev_null_) FVM_HIP(hipEventCreateWithFlags(&ev_null_, hipEventDisableTiming));
+    for (hipEvent_t& e : ev_ring_)
+        if (!e) FVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (!d_counts_) {
         FVM_HIP(hipMalloc((void**)&d_counts_, 8 * sizeof(int)));
         FVM_HIP(hipMemset(d_counts_, 0, 8 * sizeof(int)));
@@ -215,10 +228,30 @@ int Engine::retire_finished() {
         if (c && c->active && !c->finished && c->steps_total > 0 && c->steps_done >= c->steps_total) {
             c->frozen_set = current_set(*c);
             c->finished = true;
+            // marks the end of the case's last step: its read-back and release
+            // wait for this, not for the steps queued after it
+            if (!c->ev_frozen && hipEventCreateWithFlags(&c->ev_frozen, hipEventDisableTiming) != hipSuccess)
+                c->ev_frozen = nullptr;
+            if (c->ev_frozen && hipEventRecord(c->ev_frozen, stream_) != hipSuccess) {
+                (void)hipEventDestroy(c->ev_frozen);
+                c->ev_frozen = nullptr;
+            }
             n++;
         }
     if (n) lists_dirty_ = true;
     return n;
+}
+
+int Engine::throttle_begin() {
+    if (ring_used_[ring_]) FVM_HIP(hipEventSynchronize(ev_ring_[ring_]));
+    return FVM_OK;
+}
+
+int Engine::throttle_end() {
+    FVM_HIP(hipEventRecord(ev_ring_[ring_], stream_));
+    ring_used_[ring_] = true;
+    ring_ ^= 1;
+    return FVM_OK;
 }
 
 size_t Engine::bytes_in_use() const {
@@ -450,11 +483,27 @@ static bool params_valid(const fvm_params& p, std::string& why) {
     return true;
 }
 
+// Runs the admission of a batched case on the side stream: every enqueue and
+// wait of add_case / reset_case / mark_and_lists goes to stream_, which is the
+// side stream for the duration.  The admission only touches the new case's
+// memory, its descriptor slot and the engine's admission scratch lists, none
+// of which the steps queued on the main stream read.
+struct SideStream {
+    hipStream_t& s;
+    hipStream_t main;
+    SideStream(hipStream_t& stream, hipStream_t side, bool on) : s(stream), main(stream) { if (on) s = side; }
+    ~SideStream() { s = main; }
+};
+
 int Engine::add_case(const fvm_params& p, int steps_total, bool all_sets, int* id_out) {
     std::string why;
     if (!params_valid(p, why)) return set_error(FVM_ERR_INVALID_ARGUMENT, why);
     int rc = init_stream();
     if (rc) return rc;
+    // a descriptor-table growth below waits for and copies the table the main
+    // stream's steps read: never on the side stream
+    const bool side = side_admission_ && (int)cases_.size() < max_cases_;
+    SideStream on_side(stream_, side_, side);
     int id = -1;
     for (int i = 0; i < (int)cases_.size(); i++)
         if (!cases_[i]) { id = i; break; }
@@ -507,7 +556,14 @@ int Engine::add_case(const fvm_params& p, int steps_total, bool all_sets, int* i
 int Engine::remove_case(int id) {
     HostCase* hc = get(id);
     if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
-    FVM_HIP(hipStreamSynchronize(stream_));
+    // A finished case is read by no step queued after its last one (the lists
+    // are rebuilt without it): it is released once that step has run.
+    // Anything else waits for every queued step.
+    if (hc->finished && hc->ev_frozen) FVM_HIP(hipEventSynchronize(hc->ev_frozen));
+    else FVM_HIP(hipStreamSynchronize(stream_));
+    FVM_HIP(hipStreamSynchronize(side_));
+    if (hc->ev_frozen) (void)hipEventDestroy(hc->ev_frozen);
+    hc->ev_frozen = nullptr;
     if (use_arena_) {
         arena_.release(hc->mem);
         arena_.release(hc->ghost_mem);
@@ -538,8 +594,10 @@ void Engine::dev_free(void* p) {
 }
 
 // Uploads a work list into *buf (grown geometrically) and its length into
-// d_count[count_slot].  Stream-ordered: a released buffer is reused only by
-// work queued after everything that read it.
+// d_count[count_slot], in stream order after the work queued before it (which
+// still reads the old list), without waiting for that work: the copy is made
+// from a pinned staging buffer of this list.  A grown device buffer replaces
+// the old one only after everything queued has run.
 int Engine::upload_items(const std::vector<WorkItem>& v, WorkItem** buf, size_t* cap, int* d_count, int count_slot) {
     if (v.size() > *cap) {
         FVM_HIP(hipStreamSynchronize(stream_));
@@ -551,10 +609,25 @@ int Engine::upload_items(const std::vector<WorkItem>& v, WorkItem** buf, size_t*
         if (!*buf) return set_error(FVM_ERR_CAPACITY, "device memory exhausted (work list)");
         *cap = nc;
     }
-    if (!v.empty()) FVM_HIP(hipMemcpyAsync(*buf, v.data(), v.size() * sizeof(WorkItem), hipMemcpyHostToDevice, stream_));
-    int n = (int)v.size();
-    FVM_HIP(hipMemcpyAsync(d_count + count_slot, &n, sizeof(int), hipMemcpyHostToDevice, stream_));
-    FVM_HIP(hipStreamSynchronize(stream_));
+    UploadStaging& st = staging_[buf];
+    if (!st.ev) FVM_HIP(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming));
+    if (st.used) FVM_HIP(hipEventSynchronize(st.ev));
+    const size_t bytes = v.size() * sizeof(WorkItem), need = bytes + 64;
+    if (need > st.cap) {
+        if (st.host) FVM_HIP(hipHostFree(st.host));
+        st.host = nullptr;
+        st.cap = 0;
+        const size_t nc = std::max<size_t>(need * 2, 4096);
+        FVM_HIP(hipHostMalloc((void**)&st.host, nc));
+        st.cap = nc;
+    }
+    if (bytes) std::memcpy(st.host, v.data(), bytes);
+    int* const count = (int*)(st.host + st.cap - 64);
+    *count = (int)v.size();
+    if (bytes) FVM_HIP(hipMemcpyAsync(*buf, st.host, bytes, hipMemcpyHostToDevice, stream_));
+    FVM_HIP(hipMemcpyAsync(d_count + count_slot, count, sizeof(int), hipMemcpyHostToDevice, stream_));
+    FVM_HIP(hipEventRecord(st.ev, stream_));
+    st.used = true;
     return FVM_OK;
 }
 
@@ -834,6 +907,8 @@ int Engine::rebuild_lists() {
 }
 
 void Engine::drop_graphs() {
+    // an executable graph is destroyed only once no replay of it is queued
+    if ((graph_[0] || graph_[1] || graph_long_[0] || graph_long_[1]) && main_) (void)hipStreamSynchronize(main_);
     for (int i = 0; i < 2; i++) {
         if (graph_[i]) {
             (void)hipGraphExecDestroy(graph_[i]);
@@ -1098,13 +1173,18 @@ int Engine::advance(int nsteps, bool use_graph) {
 
 int Engine::sync() {
     FVM_HIP(hipStreamSynchronize(stream_));
+    if (side_) FVM_HIP(hipStreamSynchronize(side_));
     return flush_timings();
 }
 
 int Engine::read_state(int id, void* host, bool as_f64) {
     HostCase* hc = get(id);
     if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
-    FVM_HIP(hipStreamSynchronize(stream_));
+    // a finished case is read on the side stream after its last step, while
+    // the steps queued after it run on; anything else after the queued steps
+    const bool frozen = hc->finished && hc->ev_frozen;
+    hipStream_t s = frozen ? side_ : stream_;
+    if (frozen) FVM_HIP(hipStreamWaitEvent(side_, hc->ev_frozen, 0));
     CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
     int cur = current_set(*hc);
     size_t es = fp64_ ? 8 : 4;
@@ -1112,21 +1192,23 @@ int Engine::read_state(int id, void* host, bool as_f64) {
     for (int f = 0; f < 5; f++) {
         const char* src = (const char*)hc->mem + L.sets_off + L.plane * plane_index(cur, f);
         if ((as_f64 && fp64_) || (!as_f64 && !fp64_)) {
-            FVM_HIP(hipMemcpy((char*)host + f * L.n * es, src, L.n * es, hipMemcpyDeviceToHost));
+            FVM_HIP(hipMemcpyAsync((char*)host + f * L.n * es, src, L.n * es, hipMemcpyDeviceToHost, s));
         } else {
             tmp.resize(L.n * es);
-            FVM_HIP(hipMemcpy(tmp.data(), src, L.n * es, hipMemcpyDeviceToHost));
+            FVM_HIP(hipMemcpyAsync(tmp.data(), src, L.n * es, hipMemcpyDeviceToHost, s));
+            FVM_HIP(hipStreamSynchronize(s));
             if (as_f64) {
                 double* o = (double*)host + f * L.n;
-                const float* s = (const float*)tmp.data();
-                for (size_t i = 0; i < L.n; i++) o[i] = s[i];
+                const float* v = (const float*)tmp.data();
+                for (size_t i = 0; i < L.n; i++) o[i] = v[i];
             } else {
                 float* o = (float*)host + f * L.n;
-                const double* s = (const double*)tmp.data();
-                for (size_t i = 0; i < L.n; i++) o[i] = (float)s[i];
+                const double* v = (const double*)tmp.data();
+                for (size_t i = 0; i < L.n; i++) o[i] = (float)v[i];
             }
         }
     }
+    FVM_HIP(hipStreamSynchronize(s));
     return FVM_OK;
 }
 
@@ -1197,9 +1279,13 @@ int Engine::read_time(int id, double* t, long long* steps) {
     HostCase* hc = get(id);
     if (!hc) return set_error(FVM_ERR_NOT_FOUND, "no such case");
     CaseLayout L = layout_for(hc->p, fp64_, kahan_, hc->nsets);
-    FVM_HIP(hipStreamSynchronize(stream_));
-    if (t) FVM_HIP(hipMemcpy(t, (char*)hc->mem + L.t_off, 8, hipMemcpyDeviceToHost));
-    if (steps) FVM_HIP(hipMemcpy(steps, (char*)hc->mem + L.steps_off, 8, hipMemcpyDeviceToHost));
+    // a finished case: after its last step, on the side stream (as read_state)
+    const bool frozen = hc->finished && hc->ev_frozen;
+    hipStream_t s = frozen ? side_ : stream_;
+    if (frozen) FVM_HIP(hipStreamWaitEvent(side_, hc->ev_frozen, 0));
+    if (t) FVM_HIP(hipMemcpyAsync(t, (char*)hc->mem + L.t_off, 8, hipMemcpyDeviceToHost, s));
+    if (steps) FVM_HIP(hipMemcpyAsync(steps, (char*)hc->mem + L.steps_off, 8, hipMemcpyDeviceToHost, s));
+    FVM_HIP(hipStreamSynchronize(s));
     return FVM_OK;
 }
 

@@ -68,6 +68,7 @@ struct HostCase {
     long long steps_done = 0;
     bool finished = false;
     int frozen_set = -1;   // set holding the final state once finished (-1: follows the step parity)
+    hipEvent_t ev_frozen = nullptr;   // recorded after the case's last step (batch)
 };
 
 struct KernelStat {
@@ -136,6 +137,13 @@ public:
     int case_slots() const { return (int)cases_.size(); }
     void mark_lists_dirty() { lists_dirty_ = true; }
     int retire_finished();
+    // Batched serving: admissions and read-backs of finished cases on a side
+    // stream, so that neither waits for the steps queued on the main stream.
+    void set_side_admission(bool on) { side_admission_ = on; }
+    // Keeps at most two advance calls queued: waits for the one before the
+    // previous (call at the start of an advance), marks this one's end.
+    int throttle_begin();
+    int throttle_end();
     size_t bytes_in_use() const;
     int set_arena(size_t bytes);
     int reserve_cases(int n);
@@ -166,6 +174,15 @@ private:
     void* dev_alloc(size_t bytes);
     void dev_free(void* p);
     int upload_items(const std::vector<WorkItem>& v, WorkItem** buf, size_t* cap, int* d_count, int count_slot);
+    // Pinned host copy of one device list's last upload: the copy runs in
+    // stream order, so the host rewrites it only after that copy has run.
+    struct UploadStaging {
+        char* host = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+        bool used = false;
+    };
+    std::map<WorkItem**, UploadStaging> staging_;
     int timed_begin();
     // Start / stop events for the next timed launch (null when not timing).
     void timed_events(hipEvent_t* ev0, hipEvent_t* ev1);
@@ -176,6 +193,12 @@ private:
     bool fp64_, kahan_;
     int tx_ = 64, ty_ = 8;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;       // admissions and read-backs of finished cases
+    hipStream_t main_ = nullptr;       // the steps' stream (stream_ outside an admission)
+    bool side_admission_ = false;
+    hipEvent_t ev_ring_[2] = {nullptr, nullptr};
+    bool ring_used_[2] = {false, false};
+    int ring_ = 0;
     hipEvent_t ev_null_ = nullptr;   // marks the caller's null-stream work before a readback
     std::vector<HostCase*> cases_;
     int max_cases_ = 0;

@@ -57,13 +57,16 @@ class BatchEngine:
         """Capture the step graphs of the current cases without stepping."""
         check(lib().fvm_batch_prepare_graphs(self._h), "fvm_batch_prepare_graphs")
 
-    def status(self, cid: int):
+    def status(self, cid: int, with_time: bool = True):
+        """(steps done, simulated time, finished).  The step count and the
+        finished flag are host-side; with_time=False skips the device read of
+        the clock (time is then None) and so waits for nothing."""
         n = ctypes.c_int32()
         t = ctypes.c_double()
         f = ctypes.c_int32()
-        check(lib().fvm_batch_case_status(self._h, cid, ctypes.byref(n), ctypes.byref(t), ctypes.byref(f)),
-              "fvm_batch_case_status")
-        return n.value, t.value, bool(f.value)
+        check(lib().fvm_batch_case_status(self._h, cid, ctypes.byref(n), ctypes.byref(t) if with_time else None,
+                                          ctypes.byref(f)), "fvm_batch_case_status")
+        return n.value, (t.value if with_time else None), bool(f.value)
 
     def state(self, cid: int) -> np.ndarray:
         ny, nx = self._shape[cid]

@@ -8,6 +8,12 @@ has taken its step count, reads its state back and frees its memory for the
 next admission.  State never leaves HBM between steps.  A case's arithmetic
 does not depend on its neighbours in the batch, so a result is bitwise the
 same alone and in the mix.
+
+The host never waits for the device between chunks: the next chunk of steps
+is queued before the cases finished by the previous one are read back, the
+engine runs admissions and read-backs of finished cases on a side stream (a
+read-back waits for its case's last step only), and at most two chunks are
+queued at a time, so admissions join within about one chunk.
 """
 from __future__ import annotations
 
@@ -58,6 +64,7 @@ class SolveServer:
         done: list[Request] = []
         in_use = 0
         sizes: dict[int, int] = {}
+        retire: list[int] = []   # finished by the chunks queued so far, not yet read back
         t0 = time.perf_counter()
         busy = 0.0
         while True:
@@ -91,21 +98,26 @@ class SolveServer:
                     if wait > 0:
                         time.sleep(min(wait, 0.01))
                 continue
-            tb = time.perf_counter()
-            self.engine.advance(self.chunk)
-            self.engine.synchronize()
-            busy += time.perf_counter() - tb
-            for cid in list(active):
-                n, _, fin = self.engine.status(cid)
-                if fin:
-                    r = active.pop(cid)
-                    state = self.engine.state(cid)
-                    if self.keep_results:
-                        r.result = state
-                    r.completed = time.perf_counter() - t0
-                    self.engine.remove(cid)
-                    in_use -= sizes.pop(cid)
-                    done.append(r)
+            running = [cid for cid in active if cid not in retire]
+            if running:
+                # queue the next chunk first (returns once at most one earlier
+                # chunk is still queued): the device stays busy while the
+                # host reads back and admits
+                tb = time.perf_counter()
+                self.engine.advance(self.chunk)
+                busy += time.perf_counter() - tb
+            # cases finished by the chunks queued before this one
+            for cid in retire:
+                r = active.pop(cid)
+                state = self.engine.state(cid)
+                if self.keep_results:
+                    r.result = state
+                r.completed = time.perf_counter() - t0
+                self.engine.remove(cid)
+                in_use -= sizes.pop(cid)
+                done.append(r)
+            retire = [cid for cid in active if self.engine.status(cid, with_time=False)[2]]
+        self.engine.synchronize()
         wall = time.perf_counter() - t0
         lat = np.array([r.completed - r.arrival for r in done]) if done else np.zeros(1)
         work = sum(r.cells * r.steps for r in done)
@@ -115,7 +127,7 @@ class SolveServer:
             "left_in_queue": len(queue) + len(future),
             "left_active": len(active),
             "wall_s": wall,
-            "busy_s": busy,
+            "host_wait_in_advance_s": busy,
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()),
