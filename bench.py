@@ -59,8 +59,10 @@ def parse():
                     help="requests/s of the served stream (8/s offers ~8.3e9 cell-updates/s, about 75%% of capacity)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-step-scale", type=float, default=1.0, help="scales the 500-5000 step counts")
+    ap.add_argument("--serve-time-limit", type=float, default=None,
+                    help="stop the served stream after this many seconds (default 3x the window)")
     ap.add_argument("--overload-rate", type=float, default=50.0, help="requests/s of the short overload stream")
-    ap.add_argument("--overload-seconds", type=float, default=4.0)
+    ap.add_argument("--overload-seconds", type=float, default=4.0, help="0: no overload stream")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the 256x128 Euler case and the CPU mix subset")
     ap.add_argument("--direct", action="store_true",
@@ -235,7 +237,8 @@ def served(rate, seconds, step_scale, seed=5, time_limit=None):
     reqs = poisson_stream(rate, seconds, seed=seed, step_scale=step_scale)
     offered = sum(r.cells * r.steps for r in reqs) / seconds
     srv = SolveServer(0, mem_budget_bytes=200 << 30, max_active=1024, chunk_steps=25)
-    st = srv.run(reqs, time_limit=time_limit if time_limit is not None else seconds * 3)
+    st = srv.run(reqs, time_limit=time_limit if time_limit is not None else seconds * 3,
+                 progress_every=30.0 if seconds > 60 else None)
     srv.close()
     st.pop("done")
     st.update({"rate_per_s": rate, "arrival_window_s": seconds, "step_scale": step_scale,
@@ -328,10 +331,11 @@ def main():
     if not a.no_batch:
         out["batched"] = batched_vs_plain(a.batch_cases, a.batch_steps, graphs=not a.direct)
     if not a.no_serve:
-        out["served"] = served(a.serve_rate, a.serve_seconds, a.serve_step_scale)
+        out["served"] = served(a.serve_rate, a.serve_seconds, a.serve_step_scale, time_limit=a.serve_time_limit)
         # above capacity: the queue grows, latency is queueing; the run stops at 6x the window
-        out["served_overload"] = served(a.overload_rate, a.overload_seconds, a.serve_step_scale, seed=6,
-                                        time_limit=6 * a.overload_seconds)
+        if a.overload_seconds > 0:
+            out["served_overload"] = served(a.overload_rate, a.overload_seconds, a.serve_step_scale, seed=6,
+                                            time_limit=6 * a.overload_seconds)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, a.cpu_steps)
     else:

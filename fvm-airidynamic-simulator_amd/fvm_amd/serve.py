@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import collections
 import dataclasses
+import sys
 import time
 
 import numpy as np
@@ -55,9 +56,10 @@ class SolveServer:
     def close(self):
         self.engine.close()
 
-    def run(self, requests: list[Request], time_limit: float | None = None) -> dict:
+    def run(self, requests: list[Request], time_limit: float | None = None, progress_every: float | None = None) -> dict:
         """Serve `requests` (arrival times relative to the call) until all are
-        done or `time_limit` seconds pass.  Returns latency and throughput stats."""
+        done or `time_limit` seconds pass.  Returns latency and throughput stats.
+        progress_every: seconds between status lines on stderr (None: quiet)."""
         future = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue: collections.deque[Request] = collections.deque()
         active: dict[int, Request] = {}
@@ -67,10 +69,15 @@ class SolveServer:
         retire: list[int] = []   # finished by the chunks queued so far, not yet read back
         t0 = time.perf_counter()
         busy = 0.0
+        next_report = progress_every
         while True:
             now = time.perf_counter() - t0
             if time_limit is not None and now > time_limit:
                 break
+            if next_report is not None and now >= next_report:
+                next_report += progress_every
+                print(f"[serve] t={now:.0f}s arrived={len(requests) - len(future)} completed={len(done)} "
+                      f"active={len(active)} queued={len(queue)}", file=sys.stderr, flush=True)
             while future and future[0].arrival <= now:
                 queue.append(future.popleft())
             # admission: FIFO against the memory budget and the slot limit
@@ -121,6 +128,8 @@ class SolveServer:
         wall = time.perf_counter() - t0
         lat = np.array([r.completed - r.arrival for r in done]) if done else np.zeros(1)
         work = sum(r.cells * r.steps for r in done)
+        # steps taken by the cases still resident when the run stopped
+        partial = sum(r.cells * self.engine.status(cid, with_time=False)[0] for cid, r in active.items())
         return {
             "completed": len(done),
             "submitted": len(requests),
@@ -132,6 +141,7 @@ class SolveServer:
             "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()),
             "cell_updates_per_s": work / wall if wall > 0 else 0.0,
+            "executed_cell_updates_per_s": (work + partial) / wall if wall > 0 else 0.0,
             "done": done,
         }
 
