@@ -61,7 +61,10 @@ enum { FVM_FLUX_HLLC = 0, FVM_FLUX_AUSM_PLUS = 1 };
 enum { FVM_LIMITER_MINMOD = 0, FVM_LIMITER_VAN_LEER = 1 };
 /* Inviscid wall: reference slip/no-slip blend (solver.cu:2146-2173) or pure slip. */
 enum { FVM_WALL_REFERENCE = 0, FVM_WALL_SLIP = 1 };
-/* Arithmetic of the state and the update. */
+/* Arithmetic of the state and the update.  FVM_PRECISION_FP32_KAHAN (FP32
+ * fluxes, compensated SSP blend, FP64 CFL maxima and dt) is for inviscid cases:
+ * fvm_initialize, fvm_reset, fvm_step and fvm_advance reject it with enable_viscosity set
+ * (FVM_ERR_INVALID_ARGUMENT, state unchanged). */
 enum { FVM_PRECISION_FP32 = 0, FVM_PRECISION_FP32_KAHAN = 1, FVM_PRECISION_FP64 = 2 };
 /* Where dt comes from: params.dt (reference step()), or the CFL limit of the
  * current state recomputed on the device before every step. */

@@ -158,8 +158,8 @@ public:
     void computeVelocityMagToDevice(float* d) { FVM_CHECK(fvm_field_to_device(h_, FVM_FIELD_VELOCITY_MAG, d)); }
     void computeMachToDevice(float* d) { FVM_CHECK(fvm_field_to_device(h_, FVM_FIELD_MACH, d)); }
     // Velocity-arrow vertices for the OpenGL overlay (src/solver.cu:3809-3845)
-    // belong to the display path, which this library does not carry; returns -1
-    // ("failed"), the value the reference documents for failure.
+    // into the caller's mapped VBO; returns the number of vertices written
+    // (at most max_vertices).
     int generateVectorArrows(float* dev_vertex_data, int max_vertices, int step, float u_inf, float max_arrow_length,
                              float arrow_head_angle, float arrow_head_length) {
         int n = 0;

@@ -130,3 +130,35 @@ def test_fp64_device_cfl_matches_oracle_dt():
         for l1, _ in rel_errors(s.state64(), o.state()):
             assert l1 < 1e-10
     o.close()
+
+
+
+def test_kahan_rejects_viscous_cases():
+    """The compensated variant is the precision variant of the inviscid
+    configuration: a viscous case is refused at initialize, and a step call
+    that switches viscosity on is refused before any work is queued.  The
+    state is then unchanged, and the run continues bit for bit as one that
+    never made the call."""
+    p = P.euler_bow_shock(nx=128, ny=64, radius=8.0, center=(32.0, 32.0))
+    p.dt = 0.1
+    p.precision = P.PRECISION_FP32_KAHAN
+    announce("compensated variant: viscous cases rejected, state unchanged")
+    with fvm_amd.CFDSolver(0) as s:
+        q = p.copy()
+        q.enable_viscosity = 1
+        with pytest.raises(fvm_amd.FvmError, match="inviscid"):
+            s.initialize(q)
+        s.initialize(p)
+        a = p.copy()
+        s.advance(a, 10)
+        before = s.state()
+        with pytest.raises(fvm_amd.FvmError, match="inviscid"):
+            s.advance(q, 5)
+        assert np.array_equal(s.state(), before)
+        assert s.time()[1] == 10
+        s.advance(a, 10)
+        after = s.state()
+        s.initialize(p)
+        b = p.copy()
+        s.advance(b, 20)
+        assert np.array_equal(s.state(), after)
