@@ -12,12 +12,14 @@ relative to the field's scale.
   2251-2511): with the flow uniform near every domain edge, the flux
   differences and the central viscous divergences telescope, so the domain
   totals of rho, rho_u, rho_v and E stay fixed to rounding.
-- Characteristic outflow (applyBoundaryConditionsKernel 1891-2030, Riemann
-  invariants 680-841): edge cells get no normal flux difference, and where the
-  normal velocity is supersonic outwards every invariant is taken from the
-  cell itself, so a state that is uniform along the edge stays unchanged.
-  Subsonic outflow takes the incoming invariant from the freestream and
-  changes it.
+- Characteristic boundaries (applyBoundaryConditionsKernel 1891-2030, Riemann
+  invariants 680-841): for a state uniform along the edge, the edge cells get
+  no flux difference, so one step leaves them B(U0/2 + B(U0)/2), with B the
+  textbook characteristic boundary in the outward-normal frame (entropy and
+  tangential velocity at speed w, J+- = w +- 2c/(gamma-1) at w +- c, each from
+  the cell where its speed points out and from the freestream otherwise).
+  Where the normal velocity is supersonic outwards every invariant is taken
+  from the cell itself, so the edge stays unchanged.
 - Flat walls (ghost fill 2045-2225): on the flat sides of an unrotated capsule
   the SDF normal is exactly +-y, so a ghost cell takes the reflection of the
   cell directly outside it (the state before the boundary pass, also when that
@@ -100,23 +102,32 @@ def bump_moved(s0, s1):
 OUTFLOW_SIDES = ("right", "top", "bottom")
 
 
-def outflow_case(side, supersonic, f32=True):
+def outflow_case(side, supersonic, f32=True, inward=False):
     """A state that varies only across `side` and moves out through it at
-    normal Mach ~3 (supersonic) or ~0.5 (subsonic); 64x48, no obstacle."""
+    normal Mach ~3 (supersonic) or ~0.5 (subsonic), or in through it at Mach
+    ~0.5 (inward); 64x48, no obstacle."""
     speed = 3.0 if supersonic else 0.5
-    p = P.euler_bow_shock(nx=64, ny=48, mach=speed if side == "right" else 0.5)
+    p = P.euler_bow_shock(nx=64, ny=48, mach=speed if side == "right" and not inward else 0.5)
     no_obstacle(p)
     p.dt = 0.05
     X, Y = grid(p)
     s = X if side == "right" else Y
     rho = 1.0 + 0.05 * np.sin(0.37 * s)
     a = speed + 0.1 * np.cos(0.23 * s)
+    if inward:
+        a = -a
     pr = p.p_inf * (1.0 + 0.05 * np.cos(0.5 * s))
     if side == "right":
         u, v = a, np.zeros_like(a)
     else:
         u, v = np.full_like(a, 0.2), (a if side == "top" else -a)
     return p, conservative(p, rho, u, v, pr, f32)
+
+
+# (supersonic, inward) of the characteristic-boundary closed form: every
+# invariant from the cell; the incoming one from the freestream; entropy and
+# tangential velocity from the freestream as well (subsonic inflow)
+CHARACTERISTIC_KINDS = {"subsonic_out": (False, False), "supersonic_out": (True, False), "subsonic_in": (False, True)}
 
 
 def outflow_edge(p, side, margin=8):
@@ -140,6 +151,73 @@ def outflow_change(p, side, s0, s1):
         worst = max(worst, np.abs(b[c] - a[c]).max() / scale)
     ke = 0.5 * (a[1] ** 2 + a[2] ** 2) / a[0]
     worst = max(worst, np.abs(b[4] - (a[3] - ke)).max() / np.abs(a[3] - ke).max())
+    return worst
+
+
+def characteristic_state(p, rho, un, ut, pr, sign, un_far, ut_far):
+    """Characteristic boundary state, textbook form in the outward-normal
+    frame: w = sign * un is the outward normal velocity; the entropy
+    s = p / rho^gamma and the tangential velocity travel at w, the invariants
+    J+ = w + 2c/(gamma-1) at w + c and J- = w - 2c/(gamma-1) at w - c.  Each
+    is taken from the cell where its speed points out of the domain and from
+    the freestream otherwise; then w = (J+ + J-)/2, c = (gamma-1)(J+ - J-)/4,
+    rho = (c^2 / (gamma s))^(1/(gamma-1)), p = rho c^2 / gamma."""
+    g = p.gamma
+    gm1 = g - 1.0
+    c = np.sqrt(g * pr / rho)
+    c_far = math.sqrt(g * p.p_inf / p.rho_inf)
+    w, w_far = sign * un, sign * un_far
+    jp = np.where(w + c > 0, w + 2.0 * c / gm1, w_far + 2.0 * c_far / gm1)
+    jm = np.where(w - c > 0, w - 2.0 * c / gm1, w_far - 2.0 * c_far / gm1)
+    out = w > 0
+    s = np.where(out, pr / rho ** g, p.p_inf / p.rho_inf ** g)
+    utb = np.where(out, ut, ut_far)
+    wb = 0.5 * (jp + jm)
+    cb = 0.25 * gm1 * (jp - jm)
+    rb = (cb * cb / (g * s)) ** (1.0 / gm1)
+    return rb, sign * wb, utb, rb * cb * cb / g
+
+
+def edge_bc(p, side, U):
+    """The characteristic boundary of `side` applied to conserved states U
+    (5, ...)."""
+    g = p.gamma
+    rho = U[0]
+    u, v = U[1] / rho, U[2] / rho
+    pr = (g - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v))
+    if side == "right":
+        r, un, ut, pb = characteristic_state(p, rho, u, v, pr, 1.0, p.u_inf, p.v_inf)
+        ub, vb = un, ut
+    else:
+        sign = 1.0 if side == "top" else -1.0
+        r, un, ut, pb = characteristic_state(p, rho, v, u, pr, sign, p.v_inf, p.u_inf)
+        ub, vb = ut, un
+    e = pb / (g - 1.0)
+    return np.stack([r, r * ub, r * vb, e + 0.5 * r * (ub * ub + vb * vb), e])
+
+
+def outflow_expected(p, side, s0):
+    """The edge cells after one step of a state that varies only across the
+    edge (flowing out or in).  Their flux differences along the edge vanish and the normal ones
+    are not applied on the edge (updateKernel, solver.cu:1761-1768), so stage 1
+    leaves the BC of the old state, U1 = B(U0), stage 2 blends and applies it
+    again: U2 = B(U0/2 + U1/2) (solver.cu:949-967, 1925-2030)."""
+    idx = outflow_edge(p, side)
+    u0 = s0[idx]
+    u1 = edge_bc(p, side, u0)
+    return edge_bc(p, side, 0.5 * u0 + 0.5 * u1)
+
+
+def outflow_error(p, side, s0, s1):
+    """Largest error of the edge cells against outflow_expected, per field
+    relative to the field's largest magnitude on the edge (rho_u, rho_v at
+    least 1)."""
+    want = outflow_expected(p, side, s0)
+    got = s1[outflow_edge(p, side)]
+    worst = 0.0
+    for c in range(5):
+        scale = max(np.abs(want[c]).max(), 1.0 if c in (1, 2) else 0.0)
+        worst = max(worst, float(np.abs(got[c] - want[c]).max() / scale))
     return worst
 
 

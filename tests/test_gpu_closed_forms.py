@@ -1,6 +1,6 @@
 """The HIP path, through the C ABI, against the closed forms of closed_forms.py:
-conservation, characteristic outflow, flat-wall reflection, Sutherland's law
-and the non-finite fallback.  The same checks pin the oracle
+conservation, characteristic boundaries, flat-wall reflection, Sutherland's
+law and the non-finite fallback.  The same checks pin the oracle
 (test_oracle_closed_forms.py); here they hold for the GPU's FP32, Kahan and
 FP64 arithmetic, with bars set by each one's rounding."""
 import numpy as np
@@ -66,6 +66,24 @@ def test_supersonic_outflow_edge_unchanged(solver, side, precision):
             assert change < (1e-12 if precision == "fp64" else 2e-6), change
         else:
             assert change > 1e-3, change
+
+
+@pytest.mark.parametrize("precision", ["fp32", "kahan", "fp64"])
+@pytest.mark.parametrize("kind", list(C.CHARACTERISTIC_KINDS))
+@pytest.mark.parametrize("side", C.OUTFLOW_SIDES)
+def test_characteristic_edge_closed_form(solver, side, kind, precision):
+    """After one step the edge cells equal B(U0/2 + B(U0)/2), B the textbook
+    characteristic boundary (outflow at Mach ~0.5 and ~3, inflow at ~0.5).
+    FP32 bar: the boundary takes rho from (c^2 / (gamma s))^(1/(gamma-1)), which
+    amplifies the FP32 rounding of c and s about fivefold, twice over."""
+    supersonic, inward = C.CHARACTERISTIC_KINDS[kind]
+    p, s0 = C.outflow_case(side, supersonic, inward=inward)
+    announce(f"characteristic edge {side} {kind} {precision} 64x48 1 step")
+    start(solver, p, s0, precision)
+    solver.advance(p.copy(), 1)
+    err = C.outflow_error(p, side, s0, solver.state64())
+    print(f"  edge error {err:.1e}", flush=True)
+    assert err < (1e-12 if precision == "fp64" else 2e-5), err
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])

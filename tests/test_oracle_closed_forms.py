@@ -38,6 +38,24 @@ def test_supersonic_outflow_edge_unchanged(side):
         o.close()
 
 
+@pytest.mark.parametrize("side", C.OUTFLOW_SIDES)
+@pytest.mark.parametrize("kind", list(C.CHARACTERISTIC_KINDS))
+def test_characteristic_edge_closed_form(side, kind):
+    """Characteristic boundaries (solver.cu:680-841, 1925-2030): outflow at
+    Mach ~0.5 (the incoming invariant from the freestream) and ~3 (all from
+    the cell), inflow at Mach ~0.5 (entropy and tangential velocity from the
+    freestream too).  After one step the edge cells equal B(U0/2 + B(U0)/2),
+    B the textbook characteristic boundary in the outward-normal frame."""
+    supersonic, inward = C.CHARACTERISTIC_KINDS[kind]
+    p, s0 = C.outflow_case(side, supersonic, f32=False, inward=inward)
+    o = O.Oracle(p, fp64=True)
+    o.set_state(s0)
+    o.step(1, p.copy())
+    err = C.outflow_error(p, side, s0, o.state())
+    o.close()
+    assert err < 1e-12, err
+
+
 @pytest.mark.parametrize("mode", C.FLAT_WALL_MODES)
 def test_flat_wall_ghost_reflection(mode):
     p, s0 = C.flat_wall_case(mode, f32=False)
