@@ -10,7 +10,9 @@ noise.  A step is one full time step of every cell; the step sequence is
 replayed from a captured HIP graph with the state resident in HBM.
 
 Prints ONE JSON line.  `value` = cell-updates/s over exactly --steps timed
-steps (after --warmup untimed ones), bracketed by device synchronisation.
+steps, bracketed by device synchronisation.  Before them run untimed steps
+for --settle seconds (the clocks ramp up; the count is `clock_settle_steps`)
+and then --warmup untimed steps.
 `roofline` = the dominant kernel's algorithmic HBM bytes per launch over its
 average launch duration, measured with HIP events on the solver's stream in a
 timing pass right after the timed region.  `cpu_baseline` = the CPU oracle
@@ -47,6 +49,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--settle", type=float, default=0.5,
+                    help="seconds of untimed steps before the warm-up steps, for the clocks to ramp up")
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=200, help="steps of the CPU baseline sample")
@@ -266,6 +270,16 @@ def main():
     # whatever --warmup is, the timed window holds no graph capture
     if not a.direct:
         s.prepare_graphs(q)
+    # The GPU's clocks ramp up over the first few hundred steps after start-up
+    # (a 64x7 stage kernel took 50.8 us after 20 steps and 44.9 us after 400),
+    # so untimed steps run for --settle seconds before the --warmup steps;
+    # the line reports how many.
+    settle_steps = 0
+    t_settle = time.perf_counter()
+    while time.perf_counter() - t_settle < a.settle:
+        s.advance(q, 100)
+        s.synchronize()
+        settle_steps += 100
     s.advance(q, a.warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -312,6 +326,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
+        "clock_settle_steps": settle_steps,
         "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
