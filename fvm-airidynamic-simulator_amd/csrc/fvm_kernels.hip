@@ -1448,7 +1448,7 @@ struct Launch {
 
 // Tile shapes compiled for the FP32 kernels (the autotuner chooses among them)
 // and for the FP64 kernels.
-#define FVM_TILE_TABLE(X) X(64, 4) X(64, 6) X(64, 7) X(64, 8) X(64, 12) X(64, 16) X(32, 4) X(32, 8) X(32, 16) X(128, 4) X(128, 8) X(96, 8)
+#define FVM_TILE_TABLE(X) X(64, 4) X(64, 6) X(64, 7) X(64, 8) X(64, 12) X(64, 16) X(32, 4) X(32, 7) X(32, 8) X(32, 16) X(128, 4) X(128, 8) X(96, 8)
 #define FVM_TILE_TABLE64(X) X(64, 8) X(32, 8)
 
 template <class T>
