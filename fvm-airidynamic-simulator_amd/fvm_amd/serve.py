@@ -38,6 +38,7 @@ class Request:
     admitted: float = -1.0
     completed: float = -1.0
     result: np.ndarray | None = None
+    rejected: str = ""       # why a request that can never be admitted was refused
 
     @property
     def cells(self) -> int:
@@ -64,6 +65,7 @@ class SolveServer:
         queue: collections.deque[Request] = collections.deque()
         active: dict[int, Request] = {}
         done: list[Request] = []
+        rejected: list[Request] = []
         in_use = 0
         sizes: dict[int, int] = {}
         retire: list[int] = []   # finished by the chunks queued so far, not yet read back
@@ -80,17 +82,26 @@ class SolveServer:
                       f"active={len(active)} queued={len(queue)}", file=sys.stderr, flush=True)
             while future and future[0].arrival <= now:
                 queue.append(future.popleft())
-            # admission: FIFO against the memory budget and the slot limit
+            # admission: FIFO against the memory budget and the slot limit; a
+            # request that would not fit even alone is refused, not kept waiting
             while queue and len(active) < self.max_active:
                 r = queue[0]
                 need = BatchEngine.case_bytes(r.params) + r.cells * 2  # + ghost table allowance
+                if need > self.budget:
+                    r.rejected = f"needs {need} bytes, more than the {self.budget}-byte budget"
+                    rejected.append(queue.popleft())
+                    continue
                 if in_use + need > self.budget:
                     break
                 try:
                     cid = self.engine.add(r.params, steps_total=r.steps)
                 except FvmError as e:
-                    if e.code == -5:  # budget exhausted (fragmentation): retry after a retirement
+                    if e.code == -5 and active:  # budget exhausted (fragmentation): retry after a retirement
                         break
+                    if e.code == -5:            # an empty engine cannot hold it either
+                        r.rejected = str(e)
+                        rejected.append(queue.popleft())
+                        continue
                     raise
                 queue.popleft()
                 r.admitted = time.perf_counter() - t0
@@ -133,6 +144,7 @@ class SolveServer:
         return {
             "completed": len(done),
             "submitted": len(requests),
+            "rejected": len(rejected),
             "left_in_queue": len(queue) + len(future),
             "left_active": len(active),
             "wall_s": wall,

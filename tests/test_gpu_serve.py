@@ -44,6 +44,26 @@ def test_admission_respects_budget():
     assert stats["completed"] == 5
 
 
+@pytest.mark.timeout(120)
+def test_request_larger_than_the_budget_is_refused():
+    """A request that would not fit even in an empty engine is refused at
+    once (before, the loop waited for it forever when no time limit was
+    given); the requests around it are served."""
+    small = P.euler_bow_shock(nx=96, ny=48, radius=6.0, center=(24.0, 24.0))
+    small.dt = 0.1
+    big = P.euler_bow_shock(nx=1024, ny=512, radius=32.0, center=(256.0, 256.0))
+    big.dt = 0.1
+    budget = int((fvm_amd.batch.BatchEngine.case_bytes(small) + small.nx * small.ny * 2) * 3)
+    assert fvm_amd.batch.BatchEngine.case_bytes(big) > budget
+    reqs = [Request(0, 0.0, small.copy(), 10), Request(1, 0.0, big.copy(), 10), Request(2, 0.01, small.copy(), 10)]
+    print("\n[case] served: a request larger than the whole budget between two small ones", flush=True)
+    srv = SolveServer(0, mem_budget_bytes=budget, max_active=4, chunk_steps=5)
+    stats = srv.run(reqs)
+    srv.close()
+    assert stats["completed"] == 2 and stats["rejected"] == 1
+    assert sorted(r.rid for r in stats["done"]) == [0, 2]
+
+
 @pytest.mark.timeout(900)
 def test_served_large_requests_vs_fp64_oracle():
     """Requests at the served configuration's large end (1024x512 to
