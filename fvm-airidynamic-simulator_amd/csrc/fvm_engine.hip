@@ -197,6 +197,7 @@ int Engine::set_tile(int tx, int ty) {
 void Engine::set_fused_d1(bool on) {
     if (on == fuse_d1_) return;
     fuse_d1_ = on;
+    lists_dirty_ = true;   // the persistent grids follow the kernels the schedule launches
     drop_graphs();
 }
 
@@ -887,13 +888,21 @@ int Engine::rebuild_lists() {
     single_case_ = (!persistent_ && n_active == 1) ? last_id : -1;
     if (persistent_) {
         // fixed grids: captured graphs read the lists from device memory and
-        // stay valid unless a list buffer moved or the viscous kernels appear
-        grid_tiles_ = persistent_grid_;
-        grid_visc_ = persistent_grid_;
+        // stay valid unless a list buffer moved, the viscous kernels appear or
+        // the grids change.  Each grid is what the CUs hold at once: with
+        // more, the surplus workgroups would start their strided share of the
+        // items only when resident ones had finished theirs.
+        const int old_tiles = grid_tiles_, old_visc = grid_visc_;
+        int st = 0, df = 0;
+        if (cu_count_ <= 0 && hipDeviceGetAttribute(&cu_count_, hipDeviceAttributeMultiprocessorCount, device_) != hipSuccess)
+            cu_count_ = 0;
+        tiled_resident_per_cu(fp64_, tx_, ty_, kahan_, any_visc_ && fused_d1(), &st, &df);
+        grid_tiles_ = (st > 0 && cu_count_ > 0) ? st * cu_count_ : persistent_grid_;
+        grid_visc_ = (df > 0 && cu_count_ > 0) ? df * cu_count_ : persistent_grid_;
         grid_ghosts_ = 512;
         grid_reduce_ = 256;
         if (had_visc != any_visc_ || old_ptrs[0] != d_tiles_all_ || old_ptrs[1] != d_tiles_visc_ ||
-            old_ptrs[2] != d_ghosts_ || old_ptrs[3] != d_case_list_)
+            old_ptrs[2] != d_ghosts_ || old_ptrs[3] != d_case_list_ || old_tiles != grid_tiles_ || old_visc != grid_visc_)
             drop_graphs();
     } else {
         grid_tiles_ = std::max(1, n_tiles_);

@@ -222,7 +222,8 @@ private:
     bool any_visc_ = false;
     bool fuse_d1_ = false;
     bool persistent_ = false;
-    int persistent_grid_ = 2048;
+    int persistent_grid_ = 2048;         // persistent grid when the occupancy query fails
+    int cu_count_ = 0;
 
     int phase_ = 0;                  // step parity: inviscid current set and dt slot
     bool lists_dirty_ = true;

@@ -1489,6 +1489,38 @@ bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc) {
     return fp64 ? dispatch_tile<double>(tx, ty, kc) : dispatch_tile<float>(tx, ty, kc);
 }
 
+// Workgroups per CU that the occupancy calculator admits for a step kernel.
+template <typename F>
+static int resident(F kernel) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, NT, 0) != hipSuccess) return 0;
+    return n;
+}
+
+template <class T, int TX, int TY>
+static void tiled_residency(bool kahan, bool fused, int* stage, int* diffuse) {
+    int s = kahan ? std::min(resident(k_stage<T, TX, TY, 0, true>), resident(k_stage<T, TX, TY, 1, true>))
+                  : std::min(resident(k_stage<T, TX, TY, 0, false>), resident(k_stage<T, TX, TY, 1, false>));
+    if constexpr (fused_d1_fits<T, TX, TY>())
+        if (fused && !kahan) s = std::min(s, resident(k_stage<T, TX, TY, 2, false>));
+    *stage = s;
+    *diffuse = std::min(resident(k_diffuse<T, TX, TY, false>), resident(k_diffuse<T, TX, TY, true>));
+}
+
+bool tiled_resident_per_cu(bool fp64, int tx, int ty, bool kahan, bool fused, int* stage, int* diffuse) {
+    *stage = *diffuse = 0;
+#define FVM_RES(A, B) if (tx == A && ty == B) { tiled_residency<T, A, B>(kahan, fused, stage, diffuse); return true; }
+    if (fp64) {
+        using T = double;
+        FVM_TILE_TABLE64(FVM_RES)
+    } else {
+        using T = float;
+        FVM_TILE_TABLE(FVM_RES)
+    }
+#undef FVM_RES
+    return false;
+}
+
 bool fused_d1_supported(int tx, int ty, bool fp64) {
     if (fp64) return false;
 #define FVM_FITS(A, B) if (tx == A && ty == B) return fused_d1_fits<float, A, B>();

@@ -23,6 +23,11 @@ struct KernelCall {
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc);
 bool tile_supported(int tx, int ty, bool fp64);
+// Workgroups per CU resident at once for the stage kernels a step launches
+// (stage 1, stage 2, the fused stage 1 when `fused`) and for the two half
+// steps, from the occupancy calculator: the persistent grids launch that many
+// per CU, so no workgroup waits for another to finish its whole item list.
+bool tiled_resident_per_cu(bool fp64, int tx, int ty, bool kahan, bool fused, int* stage, int* diffuse);
 // Whether stage 1 with the first viscous half step fused in is built for a tile.
 bool fused_d1_supported(int tx, int ty, bool fp64);
 // Tiles of one item of the per-tile-maxima reduction: case_list holds one
