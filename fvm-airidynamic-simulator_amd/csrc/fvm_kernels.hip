@@ -535,15 +535,20 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
         const Roles r = visc ? rv : ri;
         const int nx = k.nx, ny = k.ny;
         const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
-        // dt is formed after the staging loads are issued (the update is its
-        // first user), so the slot's load is not ahead of them on the tile's
-        // critical path; the fused first half step needs it before staging
-        double dtd = 0.0;
+        const double dtd = dt_from_slot<DT64>(cd, r.slot_r);
+        const T dt = (T)dtd;
 
         // thread index opaque per tile (indices derived from it are not held
         // in registers across the tile loop)
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
+        if (it.index == 0 && threadIdx.x == 0) {
+            if (r.zero_slot) {
+                cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
+                if constexpr (DT64) { cd.slots64[r.slot_w * 2] = 0.0; cd.slots64[r.slot_w * 2 + 1] = 0.0; }
+            }
+            if (r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
+        }
 
         const Plane<T> in0(cd.b.U[r.in][0]), in1(cd.b.U[r.in][1]), in2(cd.b.U[r.in][2]), in3(cd.b.U[r.in][3]),
             in4(cd.b.U[r.in][4]);
@@ -559,7 +564,6 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
             T* const a_T = a_v + AX * AY;
             T* const b_f = a_T + AX * AY;
             uint8_t* const a_ct = (uint8_t*)(b_f + 5 * BXY);
-            dtd = dt_from_slot<DT64>(cd, r.slot_r);
             const T hdt = (T)(dtd * 0.5);
             for (int n = tid; n < AX * AY; n += NT) {
                 int li = n % AX, lj = n / AX;
@@ -646,15 +650,6 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
             s_r[n] = q[0]; s_u[n] = u; s_v[n] = v; s_p[n] = p; s_ct[n] = c;
         }
         __syncthreads();
-        if (!(MODE == 2 && visc)) dtd = dt_from_slot<DT64>(cd, r.slot_r);
-        const T dt = (T)dtd;
-        if (it.index == 0 && threadIdx.x == 0) {
-            if (r.zero_slot) {
-                cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
-                if constexpr (DT64) { cd.slots64[r.slot_w * 2] = 0.0; cd.slots64[r.slot_w * 2 + 1] = 0.0; }
-            }
-            if (r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
-        }
 
         // ---- x faces i+1/2 for local i in [-1, TX-1]
         for (int n = tid; n < FX; n += NT) {
@@ -1063,11 +1058,20 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
         const CaseConsts<T>& k = cd.c;
         const int nx = k.nx, ny = k.ny;
         const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
+        const double dtd = dt_from_slot<(sizeof(T) == 8)>(cd, r.slot_r);
+        const T hdt = (T)(dtd * 0.5);
         // The thread index, opaque to the compiler per tile: every index
         // derived from it is recomputed per tile rather than held in registers
         // across the tile loop (which spilled them).
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
+        if (it.index == 0 && threadIdx.x == 0) {
+            if (r.zero_slot) {
+                cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
+                if constexpr (sizeof(T) == 8) { cd.slots64[r.slot_w * 2] = 0.0; cd.slots64[r.slot_w * 2 + 1] = 0.0; }
+            }
+            if (r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
+        }
         const Plane<T> in[5] = {Plane<T>(cd.b.U[r.in][0]), Plane<T>(cd.b.U[r.in][1]), Plane<T>(cd.b.U[r.in][2]),
                                 Plane<T>(cd.b.U[r.in][3]), Plane<T>(cd.b.U[r.in][4])};
         const Plane<T> outp[5] = {Plane<T>(cd.b.U[r.out][0]), Plane<T>(cd.b.U[r.out][1]), Plane<T>(cd.b.U[r.out][2]),
@@ -1104,16 +1108,6 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
             }
         }
         __syncthreads();
-        // dt after the staging loads (the update is its first user)
-        const double dtd = dt_from_slot<(sizeof(T) == 8)>(cd, r.slot_r);
-        const T hdt = (T)(dtd * 0.5);
-        if (it.index == 0 && threadIdx.x == 0) {
-            if (r.zero_slot) {
-                cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
-                if constexpr (sizeof(T) == 8) { cd.slots64[r.slot_w * 2] = 0.0; cd.slots64[r.slot_w * 2 + 1] = 0.0; }
-            }
-            if (r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
-        }
 
         for (int n = tid; n < QX * QY; n += NT) {
             int ti = n % QX, tj = n / QX;
