@@ -542,6 +542,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
         // in registers across the tile loop)
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
+        __builtin_assume(tid >= 0 && tid < NT);   // tile indices divide as unsigned
         if (it.index == 0 && threadIdx.x == 0) {
             if (r.zero_slot) {
                 cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
@@ -1065,6 +1066,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
         // across the tile loop (which spilled them).
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
+        __builtin_assume(tid >= 0 && tid < NT);   // tile indices divide as unsigned
         if (it.index == 0 && threadIdx.x == 0) {
             if (r.zero_slot) {
                 cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
