@@ -164,6 +164,15 @@ struct Plane {
     __device__ __forceinline__ void put(unsigned i, T v) const { *(gmem<T>*)(base + i * (unsigned)sizeof(T)) = v; }
 };
 
+// Plane index of cell (i, j) on row stride nx, as one 24-bit multiply-add
+// (grid dimensions are below 2^24 and a plane holds at most 2^30 cells).  The
+// 64-bit multiply-add the compiler picks for j * nx + i reads a second
+// register as the high half of its addend; when that register is the target
+// of a load still in flight, every outstanding load is waited for first.
+__device__ __forceinline__ unsigned cell_index(int i, int j, int nx) {
+    return __umul24((unsigned)j, (unsigned)nx) + (unsigned)i;
+}
+
 template <class T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
 template <> __device__ __forceinline__ float tabs<float>(float a) { return fabsf(a); }
 template <> __device__ __forceinline__ double tabs<double>(double a) { return fabs(a); }
@@ -474,15 +483,21 @@ __device__ __forceinline__ double dt_from_slot(const CaseDesc<T>& cd, int slot) 
     const CaseConsts<T>& k = cd.c;
     const CaseBufs& b = cd.b;
     if (k.dt_mode == 0 || b.slots == nullptr) return (double)k.host_dt;
+    // both maxima in one load (the slot pair is 16-byte aligned; the empty
+    // asm keeps the compiler from splitting it and sinking the second half
+    // into the viscous branch, which waited out two memory latencies)
     if constexpr (DT64) {
-        const double ms = cd.slots64[slot * 2 + 0], mn = cd.slots64[slot * 2 + 1];
+        double2 sv = *(const double2*)(cd.slots64 + slot * 2);
+        asm volatile("" : "+v"(sv.x), "+v"(sv.y));
+        const double ms = sv.x, mn = sv.y;
         const double mdx = (double)k.min_dx;
         double dt = (double)k.cfl * mdx / (ms + (double)1e-10f);
         if (k.visc) dt = fmin(dt, (double)k.cfl_visc * mdx * mdx / (mn + (double)1e-10f));
         return dt;
     } else {
-        float ms = __uint_as_float(b.slots[slot * 2 + 0]);
-        float mn = __uint_as_float(b.slots[slot * 2 + 1]);
+        uint2 sv = *(const uint2*)(b.slots + slot * 2);
+        asm volatile("" : "+v"(sv.x), "+v"(sv.y));
+        const float ms = __uint_as_float(sv.x), mn = __uint_as_float(sv.y);
         float dt = k.cfl * k.min_dx / (ms + 1e-10f);
         if (k.visc) {
             float dtv = k.cfl_visc * k.min_dx * k.min_dx / (mn + 1e-10f);

@@ -458,6 +458,8 @@ int Engine::upload_desc(HostCase& hc) {
 
 static bool params_valid(const fvm_params& p, std::string& why) {
     if (p.nx < 3 || p.ny < 3) { why = "nx and ny must be >= 3"; return false; }
+    // the kernels form a cell's plane index with a 24-bit multiply
+    if (p.nx >= (1 << 24) || p.ny >= (1 << 24)) { why = "nx and ny must be below 16777216"; return false; }
     if ((long long)p.nx * p.ny > (1ll << 31) - 1) { why = "grid too large (nx*ny must fit in int32)"; return false; }
     // the kernels address a field plane with a 32-bit byte offset
     if ((long long)p.nx * p.ny * (p.precision == FVM_PRECISION_FP64 ? 8 : 4) > (1ll << 32)) {

@@ -570,7 +570,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
                 int li = n % AX, lj = n / AX;
                 int gi = min(max(i0 + li - 4, 0), nx - 1);
                 int gj = min(max(j0 + lj - 4, 0), ny - 1);
-                const unsigned g = (unsigned)(gj * nx + gi);
+                const unsigned g = cell_index(gi, gj, nx);
                 T u, v, p, tt;
                 primitives(k, in0[g], in1[g], in2[g], in3[g], in4[g], u, v, p, tt);
                 a_u[n] = u; a_v[n] = v; a_T[n] = tt; a_ct[n] = ctp[g];
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
                 const int li = n % SX, lj = n / SX;
                 const int ci = i0 + li - 2, cj = j0 + lj - 2;
                 const int gi = min(max(ci, 0), nx - 1), gj = min(max(cj, 0), ny - 1);
-                const unsigned g = (unsigned)(gj * nx + gi);
+                const unsigned g = cell_index(gi, gj, nx);
                 const int sa = (lj + 2) * AX + (li + 2);
                 const uint8_t c = a_ct[sa];
                 T v[5] = {in0[g], in1[g], in2[g], in3[g], in4[g]};
@@ -617,38 +617,72 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
                 if (n >= SX * SY) break;
                 s_r[n] = hq[m][0]; s_u[n] = hq[m][1]; s_v[n] = hq[m][2]; s_p[n] = hq[m][3]; s_ct[n] = hc[m];
             }
-        } else
-        // ---- stage tile + halo 2: conserved -> (rho, u, v, p) in LDS
-        for (int n = tid; n < SX * SY; n += NT) {
-            int li = n % SX, lj = n / SX;
-            int gi = min(max(i0 + li - 2, 0), nx - 1);
-            int gj = min(max(j0 + lj - 2, 0), ny - 1);
-            const unsigned g = (unsigned)(gj * nx + gi);
-            const uint8_t c = ctp[g];
-            T q[5] = {in0[g], in1[g], in2[g], in3[g], in4[g]};
-            if (MODE == 1 && c == CT_GHOST) {
-                // Ghost fill of the stage-1 result, done here instead of in a
-                // launch of its own: ghost neighbours at their pre-BC value
-                // (the blend input), others as stage 1 left them.  The tile that
-                // owns the cell stores it for the later kernels: into the stage-1
-                // set (inviscid: the final ghost fill blends it), or blended into
-                // the output (viscous: the second half step reads it).
-                ghost_value<T, false>(cd, r.in, r.blend, -1, ghost_index(cd.b, gi, gj), q);
-                const bool own = li >= 2 && li < TX + 2 && lj >= 2 && lj < TY + 2 && i0 + li - 2 < nx && j0 + lj - 2 < ny;
-                if (own) {
-                    if (!visc) {
-                        for (int f = 0; f < 5; f++) Plane<T>(cd.b.U[r.in][f]).put(g, q[f]);
-                    } else {
-                        for (int f = 0; f < 5; f++) {
-                            T bl = Plane<T>(cd.b.U[r.blend][f])[g];
-                            Plane<T>(cd.b.U[r.out][f]).put(g, T(0.5) * bl + T(0.5) * q[f]);
+        } else {
+            // ---- stage tile + halo 2: conserved -> (rho, u, v, p) in LDS.
+            // Cell n of the staged region from its loaded state q and tag c.
+            auto stage_cell = [&](int n, T q[5], uint8_t c) {
+                const int li = n % SX, lj = n / SX;
+                if (MODE == 1 && c == CT_GHOST) {
+                    const int gi = min(max(i0 + li - 2, 0), nx - 1);
+                    const int gj = min(max(j0 + lj - 2, 0), ny - 1);
+                    const unsigned g = cell_index(gi, gj, nx);
+                    // Ghost fill of the stage-1 result, done here instead of in a
+                    // launch of its own: ghost neighbours at their pre-BC value
+                    // (the blend input), others as stage 1 left them.  The tile that
+                    // owns the cell stores it for the later kernels: into the stage-1
+                    // set (inviscid: the final ghost fill blends it), or blended into
+                    // the output (viscous: the second half step reads it).
+                    ghost_value<T, false>(cd, r.in, r.blend, -1, ghost_index(cd.b, gi, gj), q);
+                    const bool own = li >= 2 && li < TX + 2 && lj >= 2 && lj < TY + 2 && i0 + li - 2 < nx && j0 + lj - 2 < ny;
+                    if (own) {
+                        if (!visc) {
+                            for (int f = 0; f < 5; f++) Plane<T>(cd.b.U[r.in][f]).put(g, q[f]);
+                        } else {
+                            for (int f = 0; f < 5; f++) {
+                                T bl = Plane<T>(cd.b.U[r.blend][f])[g];
+                                Plane<T>(cd.b.U[r.out][f]).put(g, T(0.5) * bl + T(0.5) * q[f]);
+                            }
                         }
                     }
                 }
+                T u, v, p;
+                primitives_uvp(k, q[0], q[1], q[2], q[3], q[4], u, v, p);
+                s_r[n] = q[0]; s_u[n] = u; s_v[n] = v; s_p[n] = p; s_ct[n] = c;
+            };
+            constexpr int SN = (SX * SY + NT - 1) / NT;
+            if constexpr (SN <= 2) {
+                // small tiles (on small grids one workgroup per CU, nothing to
+                // hide latency behind): every staged load issued first
+                T sq[SN][5];
+                uint8_t sc[SN];
+#pragma unroll
+                for (int m = 0; m < SN; m++) {
+                    const int n = tid + m * NT;
+                    if (n < SX * SY) {
+                        const int li = n % SX, lj = n / SX;
+                        const int gi = min(max(i0 + li - 2, 0), nx - 1);
+                        const int gj = min(max(j0 + lj - 2, 0), ny - 1);
+                        const unsigned g = cell_index(gi, gj, nx);
+                        sc[m] = ctp[g];
+                        sq[m][0] = in0[g]; sq[m][1] = in1[g]; sq[m][2] = in2[g]; sq[m][3] = in3[g]; sq[m][4] = in4[g];
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < SN; m++) {
+                    const int n = tid + m * NT;
+                    if (n < SX * SY) stage_cell(n, sq[m], sc[m]);
+                }
+            } else {
+                for (int n = tid; n < SX * SY; n += NT) {
+                    const int li = n % SX, lj = n / SX;
+                    const int gi = min(max(i0 + li - 2, 0), nx - 1);
+                    const int gj = min(max(j0 + lj - 2, 0), ny - 1);
+                    const unsigned g = cell_index(gi, gj, nx);
+                    const uint8_t c = ctp[g];
+                    T q[5] = {in0[g], in1[g], in2[g], in3[g], in4[g]};
+                    stage_cell(n, q, c);
+                }
             }
-            T u, v, p;
-            primitives_uvp(k, q[0], q[1], q[2], q[3], q[4], u, v, p);
-            s_r[n] = q[0]; s_u[n] = u; s_v[n] = v; s_p[n] = p; s_ct[n] = c;
         }
         __syncthreads();
 
@@ -724,7 +758,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
 #pragma unroll
             for (int c = 0; c < 5; c++) dfy[c] = (s_f[c][a + TX] - s_f[c][a]) * (T)k.inv_dy;
             if (gi >= nx || gj >= ny) continue;
-            const unsigned g = (unsigned)(gj * nx + gi);
+            const unsigned g = cell_index(gi, gj, nx);
             const int sc = (lj + 2) * SX + (li + 2);
             const uint8_t c = s_ct[sc];
             T v0[5] = {s_r[sc], st1[g], st2[g], st3[g], st4[g]};
@@ -1080,10 +1114,9 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
                                   Plane<T>(cd.b.U[r.out][3]), Plane<T>(cd.b.U[r.out][4])};
         const Plane<uint8_t> ctp(cd.b.ct);
 
-        // Every load of the tile is issued before the first use: the staged
-        // cells' states and tags, and the states the update below starts
-        // from, so that the tile waits for one memory latency, not one per
-        // staging pass.
+        // Every staging load of the tile (the staged cells' states and tags)
+        // is issued before the first is used, so that the tile waits for one
+        // memory latency, not one per staging pass.
         constexpr int SN = (SX * SY + NT - 1) / NT;
         T sq[SN][5];
         uint8_t sc[SN];
@@ -1094,7 +1127,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
                 int li = n % SX, lj = n / SX;
                 int gi = min(max(i0 + li - 2, 0), nx - 1);
                 int gj = min(max(j0 + lj - 2, 0), ny - 1);
-                const unsigned g = (unsigned)(gj * nx + gi);
+                const unsigned g = cell_index(gi, gj, nx);
 #pragma unroll
                 for (int f = 0; f < 5; f++) sq[m][f] = in[f][g];
                 sc[m] = ctp[g];
@@ -1126,7 +1159,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
             int li = n % TX, lj = n / TX;
             int gi = i0 + li, gj = j0 + lj;
             if (gi >= nx || gj >= ny) continue;
-            const unsigned g = (unsigned)(gj * nx + gi);
+            const unsigned g = cell_index(gi, gj, nx);
             int s = (lj + 2) * SX + (li + 2);
             int t = (lj + 1) * QX + (li + 1);
             uint8_t c = s_ct[s];

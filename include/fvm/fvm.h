@@ -151,7 +151,8 @@ int fvm_destroy(fvm_solver* s);
 /* initialize(params): allocate for nx*ny and reset (solver.cu:3102-3111).
  * FVM_ERR_INVALID_ARGUMENT when one field plane would exceed 4 GiB
  * (nx*ny*4 bytes in FP32, nx*ny*8 in FP64): the kernels address a plane with
- * a 32-bit byte offset. */
+ * a 32-bit byte offset; likewise when nx or ny is 2^24 or more (a cell's
+ * plane index is formed with a 24-bit multiply). */
 int fvm_initialize(fvm_solver* s, const fvm_params* p);
 /* resize(nx, ny): reallocate if the size changed (solver.cu:3119-3129). */
 int fvm_resize(fvm_solver* s, int nx, int ny);
