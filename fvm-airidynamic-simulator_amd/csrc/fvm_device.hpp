@@ -207,18 +207,18 @@ __device__ __forceinline__ void primitives_uvp(const CaseConsts<T>& k, T rho, T 
     p = tmax((T)k.gm1 * e, (T)k.min_p);
 }
 
-// Half of the limited MUSCL slope, i.e. the offset of a face value from the
-// cell value: minmod (solver.cu:110-115) or van Leer 2ab/(a+b).  Returned
-// halved so that van Leer needs no doubling; halving is exact, so the face
-// values are the same as from the full slope times 1/2.
+// Limited MUSCL slope: minmod (solver.cu:110-115) in full, van Leer 2ab/(a+b)
+// halved, ab/(a+b), so that it needs no doubling.  Face values are
+// q +- w * slope with w = 1/2 for minmod and 1 for van Leer (0 for a
+// first-order face), i.e. q +- (full slope)/2 either way; the halving is exact.
+// Branch-free: both candidates are formed and one selected.  Minmod picks the
+// operand of smaller magnitude, which for dL dR > 0 (same signs) equals
+// copysign(min(|dL|, |dR|), dL) without a NaN-quieting min.
 template <class T>
-__device__ __forceinline__ T half_limited_slope(int limiter, T dL, T dR) {
-    if (limiter == 1) {
-        T prod = dL * dR;
-        return prod > T(0) ? Num<T>::div(prod, dL + dR) : T(0);
-    }
-    T m = tmin(tabs(dL), tabs(dR));
-    return (dL * dR > T(0)) ? T(0.5) * copysign(m, dL) : T(0);
+__device__ __forceinline__ T limited_slope(int limiter, T dL, T dR) {
+    const T prod = dL * dR;
+    const T s = limiter == 1 ? Num<T>::div(prod, dL + dR) : (tabs(dL) < tabs(dR) ? dL : dR);
+    return prod > T(0) ? s : T(0);
 }
 
 // Solid or ghost tag (CT_SOLID = 1, CT_GHOST = 2) in one unsigned compare.
@@ -344,16 +344,16 @@ __device__ __forceinline__ void face_flux(const CaseConsts<T>& k, const T rho[4]
     }
     // second order unless a +-2 neighbour is a wall or a face cell is near vacuum
     bool first = wall_tag(ct[0]) || wall_tag(ct[3]) || rho[1] < (T)k.low_rho || rho[2] < (T)k.low_rho;
-    T h = first ? T(0) : T(1);
     int lim = k.limiter;
-    T rL = rho[1] + h * half_limited_slope(lim, rho[1] - rho[0], rho[2] - rho[1]);
-    T uL = un[1] + h * half_limited_slope(lim, un[1] - un[0], un[2] - un[1]);
-    T vL = ut[1] + h * half_limited_slope(lim, ut[1] - ut[0], ut[2] - ut[1]);
-    T pL = p[1] + h * half_limited_slope(lim, p[1] - p[0], p[2] - p[1]);
-    T rR = rho[2] - h * half_limited_slope(lim, rho[2] - rho[1], rho[3] - rho[2]);
-    T uR = un[2] - h * half_limited_slope(lim, un[2] - un[1], un[3] - un[2]);
-    T vR = ut[2] - h * half_limited_slope(lim, ut[2] - ut[1], ut[3] - ut[2]);
-    T pR = p[2] - h * half_limited_slope(lim, p[2] - p[1], p[3] - p[2]);
+    const T h = first ? T(0) : (lim == 1 ? T(1) : T(0.5));
+    T rL = rho[1] + h * limited_slope(lim, rho[1] - rho[0], rho[2] - rho[1]);
+    T uL = un[1] + h * limited_slope(lim, un[1] - un[0], un[2] - un[1]);
+    T vL = ut[1] + h * limited_slope(lim, ut[1] - ut[0], ut[2] - ut[1]);
+    T pL = p[1] + h * limited_slope(lim, p[1] - p[0], p[2] - p[1]);
+    T rR = rho[2] - h * limited_slope(lim, rho[2] - rho[1], rho[3] - rho[2]);
+    T uR = un[2] - h * limited_slope(lim, un[2] - un[1], un[3] - un[2]);
+    T vR = ut[2] - h * limited_slope(lim, ut[2] - ut[1], ut[3] - ut[2]);
+    T pR = p[2] - h * limited_slope(lim, p[2] - p[1], p[3] - p[2]);
     rL = tmax(rL, (T)k.min_rho);
     pL = tmax(pL, (T)k.min_p);
     rR = tmax(rR, (T)k.min_rho);
@@ -473,31 +473,45 @@ __device__ __forceinline__ A wave_speed_as(const CaseConsts<T>& k, const T v[5])
     return Num<A>::sqrt(u * u + vv * vv) + c;
 }
 
+// The CFL maxima pair of a slot as loaded: FP64 values, or FP32 bit patterns.
+template <bool DT64> struct SlotPair;
+template <> struct SlotPair<true> { double2 v; };
+template <> struct SlotPair<false> { uint2 v; };
+
+// Issues the load of a slot's maxima pair, both values in one load (the pair
+// is 16-byte aligned).  Kernels call it at a tile's top and form dt after
+// issuing their staging loads, so that its round trip overlaps theirs.  The
+// load is unconditional (every case's descriptor points at its slot buffer;
+// host-dt mode ignores the value): a conditional one would be merged with a
+// register copy that waits for it on the spot.
+template <bool DT64, class T>
+__device__ __forceinline__ SlotPair<DT64> load_slot_pair(const CaseDesc<T>& cd, int slot) {
+    SlotPair<DT64> sp;
+    if constexpr (DT64) sp.v = *(const double2*)(cd.slots64 + slot * 2);
+    else sp.v = *(const uint2*)(cd.b.slots + slot * 2);
+    return sp;
+}
+
 // dt from the maxima of the previous step (readTimeStepResult,
 // solver.cu:3250-3265): FP32 as in the reference, or (DT64: the FP64 state
 // and the compensated variant, whose engines set CaseDesc::dt64) FP64 from the FP64
 // maxima.  Returned in FP64; for FP32 it is the widened FP32 value, so the
 // FP32 arithmetic downstream is unchanged.
 template <bool DT64, class T>
-__device__ __forceinline__ double dt_from_slot(const CaseDesc<T>& cd, int slot) {
+__device__ __forceinline__ double dt_from_pair(const CaseDesc<T>& cd, SlotPair<DT64> sp) {
     const CaseConsts<T>& k = cd.c;
-    const CaseBufs& b = cd.b;
-    if (k.dt_mode == 0 || b.slots == nullptr) return (double)k.host_dt;
-    // both maxima in one load (the slot pair is 16-byte aligned; the empty
-    // asm keeps the compiler from splitting it and sinking the second half
-    // into the viscous branch, which waited out two memory latencies)
+    // both halves taken here: the compiler may not split the load and sink
+    // its second half into the viscous branch (a second round trip)
+    asm volatile("" : "+v"(sp.v.x), "+v"(sp.v.y));
+    if (k.dt_mode == 0) return (double)k.host_dt;
     if constexpr (DT64) {
-        double2 sv = *(const double2*)(cd.slots64 + slot * 2);
-        asm volatile("" : "+v"(sv.x), "+v"(sv.y));
-        const double ms = sv.x, mn = sv.y;
+        const double ms = sp.v.x, mn = sp.v.y;
         const double mdx = (double)k.min_dx;
         double dt = (double)k.cfl * mdx / (ms + (double)1e-10f);
         if (k.visc) dt = fmin(dt, (double)k.cfl_visc * mdx * mdx / (mn + (double)1e-10f));
         return dt;
     } else {
-        uint2 sv = *(const uint2*)(b.slots + slot * 2);
-        asm volatile("" : "+v"(sv.x), "+v"(sv.y));
-        const float ms = __uint_as_float(sv.x), mn = __uint_as_float(sv.y);
+        const float ms = __uint_as_float(sp.v.x), mn = __uint_as_float(sp.v.y);
         float dt = k.cfl * k.min_dx / (ms + 1e-10f);
         if (k.visc) {
             float dtv = k.cfl_visc * k.min_dx * k.min_dx / (mn + 1e-10f);

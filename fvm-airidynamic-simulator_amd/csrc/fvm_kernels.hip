@@ -535,8 +535,14 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
         const Roles r = visc ? rv : ri;
         const int nx = k.nx, ny = k.ny;
         const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
-        const double dtd = dt_from_slot<DT64>(cd, r.slot_r);
-        const T dt = (T)dtd;
+        // dt: the small tiles (at most two staging passes) form it once their
+        // staging loads are in flight; the larger ones at the top, where it
+        // costs a round trip but holds no registers across the staging loop
+        // (formed late, the 64x7 stage kernels ran 1.2 us slower)
+        constexpr bool LATE_DT = (SX * SY + NT - 1) / NT <= 2;
+        const SlotPair<DT64> slot_pair = load_slot_pair<DT64>(cd, r.slot_r);
+        double dtd = 0.0;
+        if constexpr (!LATE_DT) dtd = dt_from_pair<DT64>(cd, slot_pair);
 
         // thread index opaque per tile (indices derived from it are not held
         // in registers across the tile loop)
@@ -548,7 +554,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
                 cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
                 if constexpr (DT64) { cd.slots64[r.slot_w * 2] = 0.0; cd.slots64[r.slot_w * 2 + 1] = 0.0; }
             }
-            if (r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
+            if (!LATE_DT && r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
         }
 
         const Plane<T> in0(cd.b.U[r.in][0]), in1(cd.b.U[r.in][1]), in2(cd.b.U[r.in][2]), in3(cd.b.U[r.in][3]),
@@ -565,7 +571,6 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
             T* const a_T = a_v + AX * AY;
             T* const b_f = a_T + AX * AY;
             uint8_t* const a_ct = (uint8_t*)(b_f + 5 * BXY);
-            const T hdt = (T)(dtd * 0.5);
             for (int n = tid; n < AX * AY; n += NT) {
                 int li = n % AX, lj = n / AX;
                 int gi = min(max(i0 + li - 4, 0), nx - 1);
@@ -582,6 +587,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
                                     b_f, b_f + BXY, b_f + 2 * BXY, b_f + 3 * BXY, b_f + 4 * BXY);
             }
             __syncthreads();
+            const T hdt = (T)(dt_from_pair<DT64>(cd, slot_pair) * 0.5);
             // updated cells in registers until the half step's arrays are dead
             constexpr int SN = (SX * SY + NT - 1) / NT;
             T hq[SN][4];
@@ -684,6 +690,11 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
                 }
             }
         }
+        if constexpr (LATE_DT) {
+            dtd = dt_from_pair<DT64>(cd, slot_pair);
+            if (r.count_step && it.index == 0 && threadIdx.x == 0) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
+        }
+        const T dt = (T)dtd;
         __syncthreads();
 
         // ---- x faces i+1/2 for local i in [-1, TX-1]
@@ -1093,8 +1104,13 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
         const CaseConsts<T>& k = cd.c;
         const int nx = k.nx, ny = k.ny;
         const int i0 = (it.index % cd.tiles_x) * TX, j0 = (it.index / cd.tiles_x) * TY;
-        const double dtd = dt_from_slot<(sizeof(T) == 8)>(cd, r.slot_r);
-        const T hdt = (T)(dtd * 0.5);
+        // dt: the first half step forms it once its staging loads are in
+        // flight; the final one at the top (formed late it ran 0.4 us slower,
+        // the first 0.5 us faster)
+        constexpr bool LATE_DT = !FINAL;
+        const SlotPair<(sizeof(T) == 8)> slot_pair = load_slot_pair<(sizeof(T) == 8)>(cd, r.slot_r);
+        double dtd = 0.0;
+        if constexpr (!LATE_DT) dtd = dt_from_pair<(sizeof(T) == 8)>(cd, slot_pair);
         // The thread index, opaque to the compiler per tile: every index
         // derived from it is recomputed per tile rather than held in registers
         // across the tile loop (which spilled them).
@@ -1106,7 +1122,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
                 cd.b.slots[r.slot_w * 2] = 0u; cd.b.slots[r.slot_w * 2 + 1] = 0u;
                 if constexpr (sizeof(T) == 8) { cd.slots64[r.slot_w * 2] = 0.0; cd.slots64[r.slot_w * 2 + 1] = 0.0; }
             }
-            if (r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
+            if (!LATE_DT && r.count_step) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
         }
         const Plane<T> in[5] = {Plane<T>(cd.b.U[r.in][0]), Plane<T>(cd.b.U[r.in][1]), Plane<T>(cd.b.U[r.in][2]),
                                 Plane<T>(cd.b.U[r.in][3]), Plane<T>(cd.b.U[r.in][4])};
@@ -1142,6 +1158,11 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
                 s_u[n] = u; s_v[n] = v; s_T[n] = tt; s_ct[n] = sc[m];
             }
         }
+        if constexpr (LATE_DT) {
+            dtd = dt_from_pair<(sizeof(T) == 8)>(cd, slot_pair);
+            if (r.count_step && it.index == 0 && threadIdx.x == 0) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
+        }
+        const T hdt = (T)(dtd * 0.5);
         __syncthreads();
 
         for (int n = tid; n < QX * QY; n += NT) {
