@@ -211,11 +211,16 @@ __device__ __forceinline__ void primitives_uvp(const CaseConsts<T>& k, T rho, T 
 // halved, ab/(a+b), so that it needs no doubling.  Face values are
 // q +- w * slope with w = 1/2 for minmod and 1 for van Leer (0 for a
 // first-order face), i.e. q +- (full slope)/2 either way; the halving is exact.
-// Branch-free: both candidates are formed and one selected.  Minmod picks the
-// operand of smaller magnitude, which for dL dR > 0 (same signs) equals
-// copysign(min(|dL|, |dR|), dL) without a NaN-quieting min.
+// Branch-free.  Minmod in FP32 is the median of (dL, dR, 0): the operand of
+// smaller magnitude when the signs agree, else zero; adding +0 turns a -0
+// operand into the +0 the reference's form gives.  Otherwise both candidates
+// are formed and one selected, minmod as the operand of smaller magnitude
+// (for dL dR > 0 that is copysign(min(|dL|, |dR|), dL)).
 template <class T>
 __device__ __forceinline__ T limited_slope(int limiter, T dL, T dR) {
+    if constexpr (sizeof(T) == 4) {
+        if (limiter != 1) return __builtin_amdgcn_fmed3f(dL, dR, 0.0f) + 0.0f;
+    }
     const T prod = dL * dR;
     const T s = limiter == 1 ? Num<T>::div(prod, dL + dR) : (tabs(dL) < tabs(dR) ? dL : dR);
     return prod > T(0) ? s : T(0);
