@@ -13,9 +13,10 @@ Prints ONE JSON line.  `value` = cell-updates/s over exactly --steps timed
 steps, bracketed by device synchronisation.  Before them run untimed steps
 for --settle seconds (the clocks ramp up; the count is `clock_settle_steps`)
 and then --warmup untimed steps.
-`roofline` = the dominant kernel's algorithmic HBM bytes per launch over its
-average launch duration, measured with HIP events on the solver's stream in a
-timing pass right after the timed region.  `cpu_baseline` = the CPU oracle
+`roofline` = the dominant kernel's (the one moving the most algorithmic bytes
+per step) algorithmic HBM bytes per launch over its average launch duration,
+measured with HIP events on the solver's stream in a timing pass right after
+the timed region; `roofline.per_kernel` gives every kernel's figures.  `cpu_baseline` = the CPU oracle
 (FP32, OpenMP) on a bounded sample of the same workload on this host.
 """
 import argparse
@@ -296,7 +297,12 @@ def main():
     s.synchronize()
     kt = s.kernel_timings()
     s.enable_timing(False)
-    dominant = max(KERNEL_BYTES_PER_CELL, key=lambda k: kt.get(k, (0, 0))[0] * kt.get(k, (0, 0))[1])
+    # The dominant kernel: the one that moves the most algorithmic bytes per
+    # step (stage 2 of the split schedule: 61 of the step's 102 B per cell; the
+    # two stage kernels also take the most time, within a few percent of each
+    # other).  `per_kernel` gives every kernel's figures beside it.
+    ran = [k for k in KERNEL_BYTES_PER_CELL if kt.get(k, (0, 0))[1] > 0]
+    dominant = max(ran, key=lambda k: KERNEL_BYTES_PER_CELL[k] * kt[k][1])
     avg_ms = kt[dominant][0]
     alg_bytes = KERNEL_BYTES_PER_CELL[dominant] * cells
     achieved = alg_bytes / (avg_ms * 1e-3)
@@ -305,6 +311,14 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dominant,
                 "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": alg_bytes}
+    per_kernel = {}
+    for k in ran:
+        kb = KERNEL_BYTES_PER_CELL[k] * cells
+        per_kernel[k] = {"avg_ms": kt[k][0], "alg_bytes": kb, "hbm_frac": kb / (kt[k][0] * 1e-3) / HBM_PEAK}
+        vk = prof.get("valu_insts_per_dispatch", {}).get(k)
+        if vk:
+            per_kernel[k]["valu_frac"] = vk * 64 / (kt[k][0] * 1e-3) / VALU_PEAK
+    roofline["per_kernel"] = per_kernel
     # The roof that binds: vector-ALU issue.  SQ_INSTS_VALU (wave instructions
     # per launch, from the committed PMC summary of this build) x 64 lanes over
     # the same live launch duration, against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz.
