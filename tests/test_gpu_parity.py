@@ -385,7 +385,21 @@ def test_plane_over_4gib_rejected(solver):
     solver.initialize(P.euler_bow_shock())
 
 
-@pytest.mark.parametrize("nx,ny,r", [(3, 3, 0.0), (7, 5, 1.2), (65, 9, 2.5), (130, 67, 9.0)])
+@pytest.mark.parametrize("nx,ny", [(1 << 24, 3), (3, 1 << 24)])
+def test_dimension_past_24_bits_rejected(solver, nx, ny):
+    """A dimension of 2^24 cells (the plane itself fits in 4 GiB): the kernels
+    form a cell's plane index with a 24-bit multiply, so the parameter check
+    rejects it before any device allocation."""
+    p = P.euler_bow_shock(nx=nx, ny=ny, radius=1.0, center=(1.5, 1.5))
+    announce(f"grid {nx}x{ny}: a dimension of 2^24")
+    with pytest.raises(fvm_amd.FvmError) as e:
+        solver.initialize(p)
+    assert e.value.code == -1
+    assert "16777216" in str(e.value)
+    solver.initialize(P.euler_bow_shock())
+
+
+@pytest.mark.parametrize("nx,ny,r",[(3, 3, 0.0), (7, 5, 1.2), (65, 9, 2.5), (130, 67, 9.0)])
 def test_tiny_and_ragged_grids(solver, nx, ny, r):
     """Grids down to the 3x3 minimum and sizes that are not multiples of any
     tile shape (partial tiles on both axes), with and without an obstacle,
