@@ -473,14 +473,30 @@ __device__ __forceinline__ void viscous_update(const CaseConsts<T>& k, const T* 
 // KAHAN: the final blend is written as U^n + delta with a carried compensation.
 // ri / rv: roles for inviscid / viscous cases.
 // ---------------------------------------------------------------------------
+// Small tiles whose x faces and y faces each fit in half the workgroup: both
+// face sets in one pass, the x faces on the first two waves and the y faces on
+// the other two (different SIMDs, so the two chains run side by side), into
+// separate flux arrays.  Larger tiles: an x pass, then a y pass into the same
+// array.
+template <int TX, int TY>
+constexpr bool split_faces() {
+    return (TX + 1) * TY <= NT / 2 && TX * (TY + 1) <= NT / 2;
+}
+
+// Length of each flux component's LDS row in k_stage.
+template <int TX, int TY>
+constexpr int flux_row() {
+    constexpr int FX = (TX + 1) * TY, FY = TX * (TY + 1);
+    return split_faces<TX, TY>() ? FX + FY : (FX > FY ? FX : FY);
+}
+
 // LDS bytes of k_stage: staged (rho, u, v, p) + face fluxes + tags; MODE 2
 // overlays the half step's (u, v, T) + tags on the 4-cell halo and its five
 // stress / heat-flux planes on the 3-cell ring on the same space.
 template <class T, int TX, int TY, int MODE>
 constexpr int stage_lds() {
-    constexpr int SX = TX + 4, SY = TY + 4, FX = (TX + 1) * TY, FY = TX * (TY + 1);
-    constexpr int FMAX = FX > FY ? FX : FY;
-    constexpr int stage = (4 * SX * SY + 5 * FMAX) * (int)sizeof(T) + SX * SY;
+    constexpr int SX = TX + 4, SY = TY + 4;
+    constexpr int stage = (4 * SX * SY + 5 * flux_row<TX, TY>()) * (int)sizeof(T) + SX * SY;
     constexpr int AXY = (TX + 8) * (TY + 8), BXY = (TX + 6) * (TY + 6);
     constexpr int d1 = MODE == 2 ? (3 * AXY + 5 * BXY) * (int)sizeof(T) + AXY : 0;
     return ((stage > d1 ? stage : d1) + 15) / 16 * 16;
@@ -510,7 +526,9 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
                                               const int* __restrict__ count, Roles ri, Roles rv) {
     constexpr int SX = TX + 4, SY = TY + 4, CPT = (TX * TY + NT - 1) / NT;
     constexpr int FX = (TX + 1) * TY, FY = TX * (TY + 1);
-    constexpr int FMAX = FX > FY ? FX : FY;
+    constexpr bool SPLIT = split_faces<TX, TY>();
+    constexpr int FR = flux_row<TX, TY>();
+    constexpr int YO = SPLIT ? FX : 0;   // offset of the y fluxes in a flux row
     static_assert(MODE != 2 || fused_d1_fits<T, TX, TY>(), "fused half step: FP32 tiles within 64 KiB of LDS");
     __shared__ __align__(16) char smem[stage_lds<T, TX, TY, MODE>()];
     T* const s_r = (T*)smem;
@@ -518,8 +536,8 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
     T* const s_v = s_u + SX * SY;
     T* const s_p = s_v + SX * SY;
     // five flux components per face
-    T (*const s_f)[FMAX] = (T (*)[FMAX])(s_p + SX * SY);
-    uint8_t* const s_ct = (uint8_t*)(s_p + SX * SY + 5 * FMAX);
+    T (*const s_f)[FR] = (T (*)[FR])(s_p + SX * SY);
+    uint8_t* const s_ct = (uint8_t*)(s_p + SX * SY + 5 * FR);
     // CFL maxima: FP64 for the FP64 state and the compensated variant
     constexpr bool DT64 = KAHAN || sizeof(T) == 8;
     using MT = typename std::conditional<DT64, double, float>::type;
@@ -698,7 +716,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
         __syncthreads();
 
         // ---- x faces i+1/2 for local i in [-1, TX-1]
-        for (int n = tid; n < FX; n += NT) {
+        auto x_face = [&](int n) {
             int fi = n % (TX + 1) - 1, fj = n / (TX + 1);
             int gi = i0 + fi, gj = j0 + fj;
             T o[5] = {T(0), T(0), T(0), T(0), T(0)};
@@ -714,6 +732,34 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
             }
 #pragma unroll
             for (int c = 0; c < 5; c++) s_f[c][n] = o[c];
+        };
+        // ---- y faces j+1/2 for local j in [-1, TY-1]
+        auto y_face = [&](int n) {
+            int fi = n % TX, fj = n / TX - 1;
+            int gi = i0 + fi, gj = j0 + fj;
+            T o[5] = {T(0), T(0), T(0), T(0), T(0)};
+            if (gj >= 0 && gj < ny - 1 && gi < nx) {
+                int b = (fj + 1) * SX + (fi + 2);
+                T rr[4], un[4], ut[4], pp[4];
+                uint8_t cc[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    int s = b + m * SX;
+                    rr[m] = s_r[s]; un[m] = s_v[s]; ut[m] = s_u[s]; pp[m] = s_p[s]; cc[m] = s_ct[s];
+                }
+                face_flux(k, rr, un, ut, pp, cc, o);
+            }
+            s_f[0][YO + n] = o[0]; s_f[1][YO + n] = o[2]; s_f[2][YO + n] = o[1]; s_f[3][YO + n] = o[3];
+            s_f[4][YO + n] = o[4];
+        };
+        if constexpr (SPLIT) {
+            if (tid < NT / 2) {
+                if (tid < FX) x_face(tid);
+            } else if (tid - NT / 2 < FY) {
+                y_face(tid - NT / 2);
+            }
+        } else {
+            for (int n = tid; n < FX; n += NT) x_face(n);
         }
         __syncthreads();
 
@@ -731,27 +777,11 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
             mxr[q] = s_f[0][a + 1];
             mxl[q] = s_f[0][a];
         }
-        __syncthreads();
-
-        // ---- y faces j+1/2 for local j in [-1, TY-1]
-        for (int n = tid; n < FY; n += NT) {
-            int fi = n % TX, fj = n / TX - 1;
-            int gi = i0 + fi, gj = j0 + fj;
-            T o[5] = {T(0), T(0), T(0), T(0), T(0)};
-            if (gj >= 0 && gj < ny - 1 && gi < nx) {
-                int b = (fj + 1) * SX + (fi + 2);
-                T rr[4], un[4], ut[4], pp[4];
-                uint8_t cc[4];
-#pragma unroll
-                for (int m = 0; m < 4; m++) {
-                    int s = b + m * SX;
-                    rr[m] = s_r[s]; un[m] = s_v[s]; ut[m] = s_u[s]; pp[m] = s_p[s]; cc[m] = s_ct[s];
-                }
-                face_flux(k, rr, un, ut, pp, cc, o);
-            }
-            s_f[0][n] = o[0]; s_f[1][n] = o[2]; s_f[2][n] = o[1]; s_f[3][n] = o[3]; s_f[4][n] = o[4];
+        if constexpr (!SPLIT) {
+            __syncthreads();
+            for (int n = tid; n < FY; n += NT) y_face(n);
+            __syncthreads();
         }
-        __syncthreads();
 
         // the state the update starts from: set r.in, or the fused half step's
         // result stored above
@@ -767,7 +797,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
             int a = lj * TX + li;
             T dfy[5];
 #pragma unroll
-            for (int c = 0; c < 5; c++) dfy[c] = (s_f[c][a + TX] - s_f[c][a]) * (T)k.inv_dy;
+            for (int c = 0; c < 5; c++) dfy[c] = (s_f[c][YO + a + TX] - s_f[c][YO + a]) * (T)k.inv_dy;
             if (gi >= nx || gj >= ny) continue;
             const unsigned g = cell_index(gi, gj, nx);
             const int sc = (lj + 2) * SX + (li + 2);
@@ -787,8 +817,8 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
                     const T rc = tmax(s_r[sc], (T)k.min_rho);
                     const T fxr = T(2) * mxr[q] * Num<T>::rcp(rc + tmax(s_r[sc + 1], (T)k.min_rho));
                     const T fxl = T(2) * mxl[q] * Num<T>::rcp(tmax(s_r[sc - 1], (T)k.min_rho) + rc);
-                    const T fyt = T(2) * s_f[0][a + TX] * Num<T>::rcp(rc + tmax(s_r[sc + SX], (T)k.min_rho));
-                    const T fyb = T(2) * s_f[0][a] * Num<T>::rcp(tmax(s_r[sc - SX], (T)k.min_rho) + rc);
+                    const T fyt = T(2) * s_f[0][YO + a + TX] * Num<T>::rcp(rc + tmax(s_r[sc + SX], (T)k.min_rho));
+                    const T fyb = T(2) * s_f[0][YO + a] * Num<T>::rcp(tmax(s_r[sc - SX], (T)k.min_rho) + rc);
                     T divv = (fxr - fxl) * (T)k.inv_dx + (fyt - fyb) * (T)k.inv_dy;
                     src = -s_p[sc] * divv;
                 }
@@ -1504,7 +1534,7 @@ struct Launch {
 
 // Tile shapes compiled for the FP32 kernels (the autotuner chooses among them)
 // and for the FP64 kernels.
-#define FVM_TILE_TABLE(X) X(64, 4) X(64, 6) X(64, 7) X(64, 8) X(64, 12) X(64, 16) X(32, 4) X(32, 7) X(32, 8) X(32, 16) X(128, 4) X(128, 8) X(96, 8)
+#define FVM_TILE_TABLE(X) X(64, 4) X(64, 6) X(64, 7) X(64, 8) X(64, 12) X(64, 16) X(32, 2) X(32, 3) X(32, 4) X(32, 7) X(32, 8) X(32, 16) X(128, 4) X(128, 8) X(96, 8)
 #define FVM_TILE_TABLE64(X) X(64, 8) X(32, 8)
 
 template <class T>

@@ -27,7 +27,8 @@ from . import params as P
 from .batch import BatchEngine
 from .solver import CFDSolver
 
-TILES = [(64, 4), (64, 6), (64, 7), (64, 8), (64, 12), (64, 16), (32, 4), (32, 7), (32, 8), (32, 16), (128, 4), (128, 8), (96, 8)]
+TILES = [(64, 4), (64, 6), (64, 7), (64, 8), (64, 12), (64, 16), (32, 2), (32, 3), (32, 4), (32, 7), (32, 8), (32, 16), (128, 4), (128, 8),
+         (96, 8)]
 
 # (name, builder, steps): one workload per bucket of the default table
 WORKLOADS = [

@@ -287,7 +287,7 @@ int fvm_synchronize(fvm_solver* s);
 /* Grid size (getNx / getNy, solver.cuh:62-63). */
 int fvm_get_size(fvm_solver* s, int* nx, int* ny);
 /* Tile shape of the stage and viscous kernels, one of the compiled shapes
- * (64x4, 64x6, 64x7, 64x8, 64x12, 64x16, 32x4, 32x7, 32x8, 32x16, 128x4, 128x8, 96x8;
+ * (64x4, 64x6, 64x7, 64x8, 64x12, 64x16, 32x2, 32x3, 32x4, 32x7, 32x8, 32x16, 128x4, 128x8, 96x8;
  * FP64: 64x8, 32x8); 0,0 = the autotuned default for the grid. */
 int fvm_set_tile(fvm_solver* s, int tile_x, int tile_y);
 /* Schedule of a viscous step (no reference counterpart: the reference always

@@ -202,7 +202,7 @@ def test_param_changes_between_graph_runs(solver):
     assert not np.array_equal(a, solver.state())
 
 
-@pytest.mark.parametrize("tile", [(64, 4), (32, 7), (32, 8), (64, 16), (128, 4), (128, 8), (32, 16)])
+@pytest.mark.parametrize("tile", [(64, 4), (32, 2), (32, 3), (32, 7), (32, 8), (64, 16), (128, 4), (128, 8), (32, 16)])
 def test_tile_shape_bitwise_invariant(solver, tile):
     p = P.viscous_cylinder(nx=150, ny=70, diameter=12.0, reynolds=100.0, mach=0.4)
     p.dt = 0.25

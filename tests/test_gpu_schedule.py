@@ -59,6 +59,7 @@ CASES = [
     ("si_mach3_256x128_64x16", lambda: si_mach3(), (64, 16)),
     ("ragged_200x90_32x16", lambda: P.viscous_cylinder(nx=200, ny=90, diameter=14.0, reynolds=150.0), (32, 16)),
     ("ragged_200x90_32x7", lambda: P.viscous_cylinder(nx=200, ny=90, diameter=14.0, reynolds=150.0), (32, 7)),
+    ("ragged_200x90_32x3", lambda: P.viscous_cylinder(nx=200, ny=90, diameter=14.0, reynolds=150.0), (32, 3)),
 ]
 
 
