@@ -278,6 +278,14 @@ struct CaseLayout {
     size_t n, plane, sets_off, k_off, ct_off, sdf_off, slots_off, t_off, steps_off, tmax_off, tmax64_off, total;
 };
 
+// Upper bound of a case's tile count over the compiled tile shapes: the
+// per-tile CFL maxima arrays are sized by it.
+static size_t max_tiles(const fvm_params& p) {
+    int mx = 0, my = 0;
+    tile_min_dims(&mx, &my);
+    return (size_t)((p.nx + mx - 1) / mx) * (size_t)((p.ny + my - 1) / my);
+}
+
 static CaseLayout layout_for(const fvm_params& p, bool fp64, bool kahan, int nsets) {
     CaseLayout L;
     L.n = (size_t)p.nx * p.ny;
@@ -297,12 +305,11 @@ static CaseLayout layout_for(const fvm_params& p, bool fp64, bool kahan, int nse
     off += 256;
     L.steps_off = off;
     off += 256;
-    // per-tile CFL maxima: every compiled tile shape has tile_x >= 32 and
-    // tile_y >= 4, so ceil(nx/32) * ceil(ny/4) bounds the tile count
+    // per-tile CFL maxima, room for the tile count of any compiled shape
     L.tmax_off = off;
-    off += round256((size_t)((p.nx + 31) / 32) * ((p.ny + 3) / 4) * 2 * sizeof(float));
+    off += round256(max_tiles(p) * 2 * sizeof(float));
     L.tmax64_off = off;
-    off += round256((size_t)((p.nx + 31) / 32) * ((p.ny + 3) / 4) * 2 * sizeof(double));
+    off += round256(max_tiles(p) * 2 * sizeof(double));
     L.total = off;
     return L;
 }
@@ -409,6 +416,9 @@ int Engine::build_consts(HostCase& hc, const fvm_params& p, void* desc_out) {
 }
 
 int Engine::upload_desc(HostCase& hc) {
+    // the kernels store one maximum pair per tile into arrays of max_tiles()
+    if ((size_t)hc.tiles_x * hc.tiles_y > max_tiles(hc.p))
+        return set_error(FVM_ERR_INVALID_ARGUMENT, "tile count exceeds the per-tile maxima arrays");
     char* hd = h_descs_.data() + (size_t)hc.id * desc_size_;
     char* base = (char*)hc.mem;
     CaseLayout L = layout_for(hc.p, fp64_, kahan_, hc.nsets);

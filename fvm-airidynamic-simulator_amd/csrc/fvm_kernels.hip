@@ -1615,6 +1615,18 @@ bool fused_d1_supported(int tx, int ty, bool fp64) {
     return false;
 }
 
+// Smallest tile width and height among the compiled shapes (FP32 and FP64):
+// ceil(nx / min width) * ceil(ny / min height) bounds any shape's tile count.
+void tile_min_dims(int* tx, int* ty) {
+    int mx = 1 << 30, my = 1 << 30;
+#define FVM_MIN(A, B) mx = A < mx ? A : mx; my = B < my ? B : my;
+    FVM_TILE_TABLE(FVM_MIN)
+    FVM_TILE_TABLE64(FVM_MIN)
+#undef FVM_MIN
+    *tx = mx;
+    *ty = my;
+}
+
 bool tile_supported(int tx, int ty, bool fp64) {
 #define FVM_HAS(A, B) if (tx == A && ty == B) return true;
     if (fp64) {

@@ -23,6 +23,8 @@ struct KernelCall {
 
 bool launch_tiled(bool fp64, int tx, int ty, const KernelCall& kc);
 bool tile_supported(int tx, int ty, bool fp64);
+// Smallest tile width and height among the compiled shapes.
+void tile_min_dims(int* tx, int* ty);
 // Workgroups per CU resident at once for the stage kernels a step launches
 // (stage 1, stage 2, the fused stage 1 when `fused`) and for the two half
 // steps, from the occupancy calculator: the persistent grids launch that many

@@ -221,6 +221,28 @@ def test_tile_shape_bitwise_invariant(solver, tile):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("tile", [(32, 2), (32, 3)])
+def test_smallest_tiles_device_dt_in_fresh_solvers(tile):
+    """The shortest tiles (most tiles per grid, so the longest per-tile
+    maxima arrays) with the device CFL dt and its FP64 maxima (the compensated
+    variant): two fresh solvers, each with the tile forced, and the 64x8 run
+    agree bit for bit over 40 steps."""
+    p = P.euler_bow_shock(nx=150, ny=70, radius=10.0, center=(40.0, 35.0))
+    p.precision = P.PRECISION_FP32_KAHAN
+    p.dt_mode = P.DT_DEVICE_CFL
+    announce(f"Kahan, device dt, tile {tile} in two fresh solvers vs 64x8, 40 steps")
+    states = []
+    for t in (tile, (64, 8), tile):
+        s = fvm_amd.CFDSolver(0)
+        s.set_tile(*t)
+        s.initialize(p)
+        s.advance(p.copy(), 40)
+        states.append((s.state(), s.time()))
+        s.close()
+    for st, tm in states[1:]:
+        assert np.array_equal(st, states[0][0]) and tm == states[0][1]
+
+
 def test_kahan_variant_not_worse(solver):
     p = P.euler_bow_shock()
     p.dt = 0.1
