@@ -445,11 +445,27 @@ __device__ __forceinline__ bool bc_pointwise(const CaseConsts<T>& k, int i, int 
     return touched;
 }
 
-// Sutherland viscosity (solver.cu:2317-2319)
+// Sutherland viscosity (solver.cu:2317-2319).  i_tref = rcp(t_ref), formed
+// once per tile by the kernels that evaluate it per cell (the compiler does
+// not hoist the reciprocal of the constant out of their loops).
+template <class T>
+__device__ __forceinline__ T sutherland(const CaseConsts<T>& k, T Tv, T i_tref) {
+    T tr = Tv * i_tref;
+    return (T)k.mu_ref * (tr * Num<T>::sqrt(tr)) * ((T)k.t_ref + (T)k.s_su) * Num<T>::rcp(Tv + (T)k.s_su);
+}
 template <class T>
 __device__ __forceinline__ T sutherland(const CaseConsts<T>& k, T Tv) {
-    T tr = Tv * Num<T>::rcp((T)k.t_ref);
-    return (T)k.mu_ref * (tr * Num<T>::sqrt(tr)) * ((T)k.t_ref + (T)k.s_su) * Num<T>::rcp(Tv + (T)k.s_su);
+    return sutherland(k, Tv, Num<T>::rcp((T)k.t_ref));
+}
+
+// The per-tile constants of the viscous terms: rcp(t_ref) and rcp(Pr).
+template <class T>
+struct ViscConsts {
+    T i_tref, i_pr;
+};
+template <class T>
+__device__ __forceinline__ ViscConsts<T> visc_consts(const CaseConsts<T>& k) {
+    return ViscConsts<T>{Num<T>::rcp((T)k.t_ref), Num<T>::rcp((T)k.prandtl)};
 }
 
 // CFL wave speed |v| + c of a state (computeWaveSpeedKernel, solver.cu:2729-2739)

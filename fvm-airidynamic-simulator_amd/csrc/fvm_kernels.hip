@@ -397,9 +397,9 @@ __device__ void ghost_value(const CaseDesc<T>& cd, int src, int pre0, int pre1, 
 // central gradients of the staged u, v, T (row stride W, cell s), one-sided at
 // the domain edges (2360-2388); zero outside the domain and in solid cells.
 template <class T, int W>
-__device__ __forceinline__ void viscous_flux(const CaseConsts<T>& k, const T* s_u, const T* s_v, const T* s_T,
-                                             const uint8_t* s_ct, int s, int gi, int gj, int n, T* s_txx, T* s_tyy,
-                                             T* s_txy, T* s_qx, T* s_qy) {
+__device__ __forceinline__ void viscous_flux(const CaseConsts<T>& k, const ViscConsts<T>& vc, const T* s_u, const T* s_v,
+                                             const T* s_T, const uint8_t* s_ct, int s, int gi, int gj, int n, T* s_txx,
+                                             T* s_tyy, T* s_txy, T* s_qx, T* s_qy) {
     const int nx = k.nx, ny = k.ny;
     if (gi < 0 || gi >= nx || gj < 0 || gj >= ny || s_ct[s] == CT_SOLID) {
         s_txx[n] = T(0); s_tyy[n] = T(0); s_txy[n] = T(0); s_qx[n] = T(0); s_qy[n] = T(0);
@@ -407,8 +407,8 @@ __device__ __forceinline__ void viscous_flux(const CaseConsts<T>& k, const T* s_
     }
     const T i2x = (T)k.inv_2dx, i2y = (T)k.inv_2dy, ix = (T)k.inv_dx, iy = (T)k.inv_dy;
     T Tv = s_T[s];
-    T mu = sutherland(k, Tv);
-    T kk = mu * (T)k.cp * Num<T>::rcp((T)k.prandtl);
+    T mu = sutherland(k, Tv, vc.i_tref);
+    T kk = mu * (T)k.cp * vc.i_pr;
     T dudx = (s_u[s + 1] - s_u[s - 1]) * i2x, dvdx = (s_v[s + 1] - s_v[s - 1]) * i2x;
     T dudy = (s_u[s + W] - s_u[s - W]) * i2y, dvdy = (s_v[s + W] - s_v[s - W]) * i2y;
     T dTdx = (s_T[s + 1] - s_T[s - 1]) * i2x, dTdy = (s_T[s + W] - s_T[s - W]) * i2y;
@@ -589,6 +589,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
             T* const a_T = a_v + AX * AY;
             T* const b_f = a_T + AX * AY;
             uint8_t* const a_ct = (uint8_t*)(b_f + 5 * BXY);
+            const ViscConsts<T> vc = visc_consts(k);
             for (int n = tid; n < AX * AY; n += NT) {
                 int li = n % AX, lj = n / AX;
                 int gi = min(max(i0 + li - 4, 0), nx - 1);
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(NT, (stage_waves<T, TX, TY, MODE, KAHAN>())) void k
             __syncthreads();
             for (int n = tid; n < BXY; n += NT) {
                 int ti = n % BX, tj = n / BX;
-                viscous_flux<T, AX>(k, a_u, a_v, a_T, a_ct, (tj + 1) * AX + (ti + 1), i0 + ti - 3, j0 + tj - 3, n,
+                viscous_flux<T, AX>(k, vc, a_u, a_v, a_T, a_ct, (tj + 1) * AX + (ti + 1), i0 + ti - 3, j0 + tj - 3, n,
                                     b_f, b_f + BXY, b_f + 2 * BXY, b_f + 3 * BXY, b_f + 4 * BXY);
             }
             __syncthreads();
@@ -1193,11 +1194,12 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
             if (r.count_step && it.index == 0 && threadIdx.x == 0) { cd.b.t[0] += dtd; cd.b.steps[0] += 1; }
         }
         const T hdt = (T)(dtd * 0.5);
+        const ViscConsts<T> vc = visc_consts(k);
         __syncthreads();
 
         for (int n = tid; n < QX * QY; n += NT) {
             int ti = n % QX, tj = n / QX;
-            viscous_flux<T, SX>(k, s_u, s_v, s_T, s_ct, (tj + 1) * SX + (ti + 1), i0 + ti - 1, j0 + tj - 1, n,
+            viscous_flux<T, SX>(k, vc, s_u, s_v, s_T, s_ct, (tj + 1) * SX + (ti + 1), i0 + ti - 1, j0 + tj - 1, n,
                                 s_txx, s_tyy, s_txy, s_qx, s_qy);
         }
         __syncthreads();
@@ -1223,7 +1225,7 @@ __global__ __launch_bounds__(NT, (diffuse_waves<T, TX, TY>())) void k_diffuse(co
                 if (c == CT_GHOST) continue;  // filled by k_ghost
                 bc_pointwise(k, gi, gj, c, v);
                 smax = tmax(smax, wave_speed_as<T>(k, v));
-                T mu = (c == CT_SOLID) ? T(0) : sutherland(k, Tin);
+                T mu = (c == CT_SOLID) ? T(0) : sutherland(k, Tin, vc.i_tref);
                 numax = tmax(numax, mu * Num<T>::rcp(v[0] + T(1e-10f)));
             }
             // the first half step leaves rho alone: set D's density plane is set A's
