@@ -509,7 +509,7 @@ constexpr bool fused_d1_fits() {
 }
 
 // Waves per SIMD asked of the register allocator for k_stage: seven for FP32
-// (at most 72 VGPRs; the 64x7 stage kernels need 63-65) and six for the
+// (at most 72 VGPRs; the 64x7 stage kernels use all 72 without spills) and six for the
 // compensated variant (80 VGPRs: at 72 it spills), unless the tile's LDS
 // already admits fewer workgroups per CU (160 KiB of LDS, one wave per SIMD
 // per 256-thread workgroup); FP64 keeps its registers.
