@@ -267,7 +267,7 @@ def main():
     s.use_graph(not a.direct)
     s.initialize(p)
     q = p.copy()
-    # the 2- and 16-step graphs the timed steps replay are captured here, so
+    # the 2-, 16- and 64-step graphs the timed steps replay are captured here, so
     # whatever --warmup is, the timed window holds no graph capture
     if not a.direct:
         s.prepare_graphs(q)

@@ -929,7 +929,8 @@ int Engine::rebuild_lists() {
 
 void Engine::drop_graphs() {
     // an executable graph is destroyed only once no replay of it is queued
-    if ((graph_[0] || graph_[1] || graph_long_[0] || graph_long_[1]) && main_) (void)hipStreamSynchronize(main_);
+    if ((graph_[0] || graph_[1] || graph_long_[0] || graph_long_[1] || graph_xl_[0] || graph_xl_[1]) && main_)
+        (void)hipStreamSynchronize(main_);
     for (int i = 0; i < 2; i++) {
         if (graph_[i]) {
             (void)hipGraphExecDestroy(graph_[i]);
@@ -939,9 +940,14 @@ void Engine::drop_graphs() {
             (void)hipGraphExecDestroy(graph_long_[i]);
             graph_long_[i] = nullptr;
         }
+        if (graph_xl_[i]) {
+            (void)hipGraphExecDestroy(graph_xl_[i]);
+            graph_xl_[i] = nullptr;
+        }
     }
     graph_valid_ = false;
     long_valid_ = false;
+    xl_valid_ = false;
 }
 
 int Engine::flush_timed_set(int set) {
@@ -1123,6 +1129,15 @@ int Engine::capture_long_graphs() {
     return FVM_OK;
 }
 
+int Engine::capture_xl_graphs() {
+    for (int g = 0; g < 2; g++) {
+        int rc = capture_sequence(g, XL_STEPS, &graph_xl_[g]);
+        if (rc) return rc;
+    }
+    xl_valid_ = true;
+    return FVM_OK;
+}
+
 void advance_counters(std::vector<HostCase*>& cases, int nsteps) {
     for (HostCase* c : cases)
         if (c && c->active && !c->finished) c->steps_done += nsteps;
@@ -1157,6 +1172,14 @@ int Engine::step_graph(int nsteps) {
         int rc = capture_long_graphs();
         if (rc) return rc;
     }
+    if (nsteps >= XL_STEPS && !xl_valid_) {
+        int rc = capture_xl_graphs();
+        if (rc) return rc;
+    }
+    while (nsteps - done >= XL_STEPS) {
+        FVM_HIP(hipGraphLaunch(graph_xl_[phase_], stream_));
+        done += XL_STEPS;
+    }
     while (nsteps - done >= LONG_STEPS) {
         FVM_HIP(hipGraphLaunch(graph_long_[phase_], stream_));
         done += LONG_STEPS;
@@ -1182,6 +1205,10 @@ int Engine::prepare_graphs() {
     }
     if (!long_valid_) {
         int rc = capture_long_graphs();
+        if (rc) return rc;
+    }
+    if (!xl_valid_) {
+        int rc = capture_xl_graphs();
         if (rc) return rc;
     }
     return FVM_OK;

@@ -101,10 +101,10 @@ public:
 
     // stepping
     int step_direct(int nsteps);        // launches without a graph
-    int step_graph(int nsteps);         // replays captured 16- and 2-step graphs
+    int step_graph(int nsteps);         // replays captured 64-, 16- and 2-step graphs
     int advance(int nsteps, bool use_graph);
     // Builds the work lists and captures the step graphs that step_graph
-    // replays (both parities, 2 and LONG_STEPS steps) without stepping, so a
+    // replays (both parities, 2, LONG_STEPS and XL_STEPS steps) without stepping, so a
     // later advance only launches.
     int prepare_graphs();
     int sync();
@@ -169,6 +169,7 @@ private:
     int enqueue_step(int phase);
     int capture_graphs();
     int capture_long_graphs();
+    int capture_xl_graphs();
     int capture_sequence(int first_phase, int nsteps, hipGraphExec_t* out);
     void drop_graphs();
     void* dev_alloc(size_t bytes);
@@ -227,15 +228,18 @@ private:
 
     int phase_ = 0;                  // step parity: inviscid current set and dt slot
     bool lists_dirty_ = true;
-    // Captured step sequences per starting parity: two steps, and LONG_STEPS
-    // steps (captured on first use), both even so the parity returns to its start.
-    // Consecutive graph launches leave a gap of ~8 us on the device, so long
-    // runs take the long graph.
+    // Captured step sequences per starting parity: two steps, LONG_STEPS and
+    // XL_STEPS steps (captured on first use), all even so the parity returns to
+    // its start.  Consecutive graph launches leave a gap of ~8 us on the
+    // device, so long runs take the longest graph that fits.
     static constexpr int LONG_STEPS = 16;
+    static constexpr int XL_STEPS = 64;
     hipGraphExec_t graph_[2] = {nullptr, nullptr};
     hipGraphExec_t graph_long_[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_xl_[2] = {nullptr, nullptr};
     bool graph_valid_ = false;
     bool long_valid_ = false;
+    bool xl_valid_ = false;
 
     Arena arena_;
     bool use_arena_ = false;

@@ -169,8 +169,8 @@ int fvm_step(fvm_solver* s, fvm_params* p);
  * replayed from a captured HIP graph and dt never leaves the device.
  * params->t_current / step are updated at the end. */
 int fvm_advance(fvm_solver* s, fvm_params* p, int nsteps);
-/* Captures the HIP graphs fvm_advance replays with FVM_DT_DEVICE_CFL (2 and
- * 16 steps, both step parities) for params *p without stepping, so the next
+/* Captures the HIP graphs fvm_advance replays with FVM_DT_DEVICE_CFL (2, 16
+ * and 64 steps, both step parities) for params *p without stepping, so the next
  * fvm_advance only launches.  A no-op with FVM_DT_HOST.  No reference
  * counterpart (the reference launches every kernel of every step). */
 int fvm_prepare_graphs(fvm_solver* s, const fvm_params* p);

@@ -167,6 +167,24 @@ def test_graph_replay_bitwise_equals_direct(solver):
     assert np.array_equal(a, b)
 
 
+def test_all_graph_lengths_bitwise_equal_direct(solver):
+    """150 steps in one advance (two 64-step graphs, one 16-step graph and
+    three 2-step graphs) against 150 single launches, bitwise, with the CFL dt
+    on the device."""
+    p = P.euler_bow_shock(nx=160, ny=96, radius=12.0, center=(40.0, 48.0))
+    p.dt_mode = P.DT_DEVICE_CFL
+    announce("64-, 16- and 2-step graph replay vs direct launches, 150 steps")
+    solver.initialize(p)
+    q = p.copy()
+    solver.advance(q, 150)
+    a, ta = solver.state(), solver.time()
+    solver.reset(p)
+    q = p.copy()
+    for _ in range(150):
+        solver.step(q)
+    assert np.array_equal(a, solver.state()) and ta == solver.time()
+
+
 def test_param_changes_between_graph_runs(solver):
     """Parameters changed between advance() calls (dt, then flux scheme and
     limiter) take effect in graph replay (16- and 2-step graphs) exactly as in
