@@ -185,21 +185,23 @@ def test_all_graph_lengths_bitwise_equal_direct(solver):
     assert np.array_equal(a, solver.state()) and ta == solver.time()
 
 
-def test_param_changes_between_graph_runs(solver):
+@pytest.mark.parametrize("leg", [20, 66])
+def test_param_changes_between_graph_runs(solver, leg):
     """Parameters changed between advance() calls (dt, then flux scheme and
-    limiter) take effect in graph replay (16- and 2-step graphs) exactly as in
-    single launches: three 20-step legs, bitwise."""
+    limiter) take effect in graph replay exactly as in single launches: three
+    legs, bitwise.  20-step legs replay the 16- and 2-step graphs, 66-step legs
+    the 64- and 2-step graphs."""
     p = P.euler_bow_shock(nx=160, ny=96, radius=12.0, center=(40.0, 48.0))
     p.dt = 0.1
-    announce("parameter changes between graph runs, 3 x 20 steps vs single launches")
+    announce(f"parameter changes between graph runs, 3 x {leg} steps vs single launches")
 
     def legs(run):
         q = p.copy()
-        run(q, 20)
+        run(q, leg)
         q.dt = 0.07
-        run(q, 20)
+        run(q, leg)
         q.flux_scheme, q.limiter = P.FLUX_AUSM_PLUS, P.LIMITER_VAN_LEER
-        run(q, 20)
+        run(q, leg)
 
     solver.initialize(p)
     legs(lambda q, n: solver.advance(q, n))
@@ -214,9 +216,9 @@ def test_param_changes_between_graph_runs(solver):
     b = solver.state()
     assert np.all(np.isfinite(a))
     assert np.array_equal(a, b)
-    # and the changes did take effect: 60 steps at the first parameters differ
+    # and the changes did take effect: as many steps at the first parameters differ
     solver.initialize(p)
-    solver.advance(p.copy(), 60)
+    solver.advance(p.copy(), 3 * leg)
     assert not np.array_equal(a, solver.state())
 
 
