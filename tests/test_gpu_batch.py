@@ -27,18 +27,21 @@ def alone(p, nsteps):
     return out
 
 
-def test_batch_equals_alone_bitwise():
+@pytest.mark.parametrize("nsteps", [30, 70])
+def test_batch_equals_alone_bitwise(nsteps):
+    """30 steps replay the 16- and 2-step graphs, 70 steps the 64- and 2-step
+    graphs, in the batch's persistent grids and in each solo run."""
     cases = small_mix(10, seed=11)
-    print(f"\n[case] batch of {len(cases)} mixed cases, 30 steps vs each alone", flush=True)
+    print(f"\n[case] batch of {len(cases)} mixed cases, {nsteps} steps vs each alone", flush=True)
     b = BatchEngine(0, mem_budget_bytes=2 << 30)
     ids = [b.add(p) for p in cases]
-    b.advance(30)
+    b.advance(nsteps)
     for cid, p in zip(ids, cases):
         n, t, fin = b.status(cid)
-        assert n == 30
+        assert n == nsteps
         got = b.state(cid)
         assert np.isfinite(got).all()
-        assert np.array_equal(got, alone(p, 30)), f"case {cid} differs from its solo run"
+        assert np.array_equal(got, alone(p, nsteps)), f"case {cid} differs from its solo run"
     b.close()
 
 
